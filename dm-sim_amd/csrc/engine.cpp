@@ -1,0 +1,345 @@
+// engine.cpp -- Simulation host object: rho resident in HBM, a packed device
+// gate table, runs, measurement.  Reference: src/dmsim_amdgpu_omp.hpp:208-829.
+#include "engine.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <sstream>
+
+#include "dmsim_amd.h"
+#include "fuse.hpp"
+#include "sweep.hpp"
+
+namespace dmx {
+
+namespace {
+thread_local std::string t_error;
+}
+
+void set_error(const std::string& msg) { t_error = msg; }
+const char* get_error() { return t_error.c_str(); }
+
+#define HIPTRY(call)                                                                  \
+    do {                                                                              \
+        hipError_t e_ = (call);                                                       \
+        if (e_ != hipSuccess) {                                                       \
+            set_error(std::string(#call) + ": " + hipGetErrorString(e_));             \
+            return e_ == hipErrorOutOfMemory ? DMX_ENOMEM : DMX_EHIP;                 \
+        }                                                                             \
+    } while (0)
+
+Engine::~Engine()
+{
+    delete plan_;
+    if (d_re_) (void)hipFree(d_re_);
+    if (d_im_) (void)hipFree(d_im_);
+    if (d_diag_) (void)hipFree(d_diag_);
+    if (d_ops_) (void)hipFree(d_ops_);
+    if (ev0_) (void)hipEventDestroy(ev0_);
+    if (ev1_) (void)hipEventDestroy(ev1_);
+    if (stream_) (void)hipStreamDestroy(stream_);
+}
+
+int Engine::init(uint32_t n_qubits, uint32_t n_gpus)
+{
+    if (n_gpus != 1) {
+        set_error("n_gpus must be 1: this engine runs one GPU per process");
+        return DMX_EINVAL;
+    }
+    if (n_qubits < 1 || n_qubits > 16) {
+        set_error("n_qubits must be in 1..16 (rho needs 16*4^n bytes of HBM)");
+        return DMX_EINVAL;
+    }
+    n_ = (int)n_qubits;
+    dim_ = 1ull << n_;
+    num_ = dim_ * dim_;
+    HIPTRY(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+    HIPTRY(hipEventCreate(&ev0_));
+    HIPTRY(hipEventCreate(&ev1_));
+    HIPTRY(hipMalloc(&d_re_, num_ * sizeof(double)));
+    HIPTRY(hipMalloc(&d_im_, num_ * sizeof(double)));
+    HIPTRY(hipMalloc(&d_diag_, dim_ * sizeof(double)));
+    HIPTRY(plain_persistent_grid(n_, &persistent_blocks_));
+    return reset_dm();
+}
+
+int Engine::append(const Gate& g)
+{
+    if (g.op < 0 || g.op >= OP_COUNT) {
+        set_error("unknown op code " + std::to_string(g.op));
+        return DMX_EINVAL;
+    }
+    const int k = g.op == OP_MAT ? g.k : op_arity(g.op);
+    if (g.op == OP_MAT) {
+        if (k < 1 || k > 5 || (int)g.mre.size() != (1 << (2 * k)) ||
+            (int)g.mim.size() != (1 << (2 * k))) {
+            set_error("matrix gate needs 1 <= k <= 5 and 4^k entries per plane");
+            return DMX_EINVAL;
+        }
+        if (k > n_) {
+            set_error("matrix gate wider than the register");
+            return DMX_EINVAL;
+        }
+    }
+    for (int i = 0; i < k; i++) {
+        if ((int)g.qb[i] >= n_) {
+            set_error("qubit index " + std::to_string(g.qb[i]) + " out of range for " +
+                      std::to_string(n_) + " qubits");
+            return DMX_EINVAL;
+        }
+        for (int j = 0; j < i; j++)
+            if (g.qb[i] == g.qb[j]) {
+                set_error(std::string(g.op == OP_MAT ? "MAT" : kOpNames[g.op]) +
+                          ": operands must be distinct qubits");
+                return DMX_EINVAL;
+            }
+    }
+    Gate copy = g;
+    for (int i = k; i < 5; i++) copy.qb[i] = 0;
+    circuit_.push_back(std::move(copy));
+    return DMX_OK;
+}
+
+int Engine::ensure_ops_capacity(size_t count)
+{
+    if (count <= d_ops_cap_) return DMX_OK;
+    if (d_ops_) HIPTRY(hipFree(d_ops_));
+    d_ops_ = nullptr;
+    d_ops_cap_ = 0;
+    const size_t cap = std::max<size_t>(count, 64);
+    HIPTRY(hipMalloc(&d_ops_, cap * sizeof(PrimOp)));
+    d_ops_cap_ = cap;
+    return DMX_OK;
+}
+
+int Engine::upload()
+{
+    if (uploaded_) {
+        set_error("upload() called again without clear_circuit()");
+        return DMX_ESTATE;
+    }
+    prims_.clear();
+    bool wide = false;
+    for (const Gate& g : circuit_)
+        if (!lower_gate(g, prims_)) wide = true;
+    delete plan_;
+    plan_ = nullptr;
+    if (wide && method_ != DMX_METHOD_FUSED) {
+        // a k >= 3 matrix has no primitive sweep; it needs the fused executor
+        set_error("matrix gates on 3..5 qubits need DMX_METHOD_FUSED");
+        return DMX_ESTATE;
+    }
+    if (!prims_.empty()) {
+        int rc = ensure_ops_capacity(prims_.size());
+        if (rc) return rc;
+        HIPTRY(hipMemcpyAsync(d_ops_, prims_.data(), prims_.size() * sizeof(PrimOp),
+                              hipMemcpyHostToDevice, stream_));
+    }
+    if (method_ == DMX_METHOD_FUSED) {
+        int rc = build_fused_plan(circuit_, n_, fusion_max_k_, &plan_);
+        if (rc) return rc;
+        rc = plan_->upload(stream_);
+        if (rc) return rc;
+    }
+    HIPTRY(hipStreamSynchronize(stream_));
+    uploaded_ = true;
+    return DMX_OK;
+}
+
+int Engine::run()
+{
+    if (!uploaded_) {
+        set_error("run() before upload()");
+        return DMX_ESTATE;
+    }
+    if (method_ == DMX_METHOD_FUSED) {
+        if (!plan_) {
+            set_error("circuit was uploaded under another method; clear_circuit() and upload()");
+            return DMX_ESTATE;
+        }
+        return run_fused();
+    }
+    return run_plain(method_ == DMX_METHOD_PLAIN_LAUNCH);
+}
+
+int Engine::run_plain(bool per_launch)
+{
+    const int n_ops = (int)prims_.size();
+    last_launches_ = 0;
+    last_sweeps_ = 2u * (uint32_t)n_ops;
+    HIPTRY(hipEventRecord(ev0_, stream_));
+    if (n_ops > 0) {
+        // reference orientation: fast index first; transposed: slow first
+        const bool first_slow = transposed_;
+        for (int pass = 0; pass < 2; pass++) {
+            const bool slow = pass == 0 ? first_slow : !first_slow;
+            int t = 0;
+            while (t < n_ops) {
+                if (per_launch || prims_[t].kind == PK_C2) {
+                    HIPTRY(launch_plain_one(d_re_, d_im_, d_ops_ + t, prims_[t], n_, slow,
+                                            max_blocks_, stream_));
+                    last_launches_++;
+                    t++;
+                    continue;
+                }
+                int end = t;
+                while (end < n_ops && prims_[end].kind != PK_C2) end++;
+                HIPTRY(launch_plain_persistent(d_re_, d_im_, d_ops_, t, end, n_, slow,
+                                               persistent_blocks_, stream_));
+                last_launches_++;
+                t = end;
+            }
+        }
+        transposed_ = !transposed_;
+    }
+    HIPTRY(hipEventRecord(ev1_, stream_));
+    HIPTRY(hipEventSynchronize(ev1_));
+    float ms = 0.f;
+    HIPTRY(hipEventElapsedTime(&ms, ev0_, ev1_));
+    last_ms_ = ms;
+    return DMX_OK;
+}
+
+int Engine::run_fused()
+{
+    last_launches_ = 0;
+    last_sweeps_ = 0;
+    HIPTRY(hipEventRecord(ev0_, stream_));
+    int rc = plan_->execute(d_re_, d_im_, n_, transposed_, d_ops_, stream_, persistent_blocks_,
+                            &last_launches_, &last_sweeps_);
+    if (rc) return rc;
+    HIPTRY(hipEventRecord(ev1_, stream_));
+    HIPTRY(hipEventSynchronize(ev1_));
+    float ms = 0.f;
+    HIPTRY(hipEventElapsedTime(&ms, ev0_, ev1_));
+    last_ms_ = ms;
+    return DMX_OK;
+}
+
+int Engine::clear_circuit()
+{
+    circuit_.clear();
+    prims_.clear();
+    delete plan_;
+    plan_ = nullptr;
+    uploaded_ = false;
+    return DMX_OK;
+}
+
+int Engine::reset_dm()
+{
+    HIPTRY(hipMemsetAsync(d_re_, 0, num_ * sizeof(double), stream_));
+    HIPTRY(hipMemsetAsync(d_im_, 0, num_ * sizeof(double), stream_));
+    const double one = 1.0;
+    HIPTRY(hipMemcpyAsync(d_re_, &one, sizeof(double), hipMemcpyHostToDevice, stream_));
+    HIPTRY(hipStreamSynchronize(stream_));
+    transposed_ = false;
+    return DMX_OK;
+}
+
+int Engine::get_diag(double* diag)
+{
+    HIPTRY(launch_diag(d_re_, n_, d_diag_, stream_));
+    HIPTRY(hipMemcpyAsync(diag, d_diag_, dim_ * sizeof(double), hipMemcpyDeviceToHost, stream_));
+    HIPTRY(hipStreamSynchronize(stream_));
+    return DMX_OK;
+}
+
+// measure(): src/dmsim_amdgpu_omp.hpp:536-564.  Sequential prefix sum of
+// |Re rho_ii| (same order, so the same doubles), glibc srand/rand; the shot is
+// the j with scan[j] <= r < scan[j+1], 0 if there is none.  scan is
+// non-decreasing, so that j is unique and a binary search finds it.
+int Engine::measure(uint32_t reps, uint32_t seed, uint32_t* out, double* total)
+{
+    std::vector<double> diag(dim_);
+    int rc = get_diag(diag.data());
+    if (rc) return rc;
+    std::vector<double> scan(dim_ + 1);
+    scan[0] = 0;
+    for (uint64_t i = 1; i < dim_ + 1; i++) scan[i] = scan[i - 1] + std::fabs(diag[i - 1]);
+    srand(seed);
+    for (uint32_t s = 0; s < reps; s++) {
+        const double r = (double)rand() / (double)RAND_MAX;
+        // first index with scan[idx] > r
+        const uint64_t idx =
+            (uint64_t)(std::upper_bound(scan.begin(), scan.end(), r) - scan.begin());
+        uint32_t res = 0;
+        if (idx >= 1 && idx <= dim_) res = (uint32_t)(idx - 1);
+        out[s] = res;
+    }
+    if (std::fabs(scan[dim_] - 1.0) > 1e-3)  // ERROR_BAR, src/config.hpp:53
+        printf("Sum of probability along diag is far from 1.0 with %lf\n", scan[dim_]);
+    if (total) *total = scan[dim_];
+    return DMX_OK;
+}
+
+int Engine::to_reference_orientation()
+{
+    if (!transposed_) return DMX_OK;
+    HIPTRY(launch_conj_transpose(d_re_, d_im_, n_, stream_));
+    transposed_ = false;
+    return DMX_OK;
+}
+
+int Engine::get_dm(double* re, double* im)
+{
+    int rc = to_reference_orientation();
+    if (rc) return rc;
+    if (re)
+        HIPTRY(hipMemcpyAsync(re, d_re_, num_ * sizeof(double), hipMemcpyDeviceToHost, stream_));
+    if (im)
+        HIPTRY(hipMemcpyAsync(im, d_im_, num_ * sizeof(double), hipMemcpyDeviceToHost, stream_));
+    HIPTRY(hipStreamSynchronize(stream_));
+    return DMX_OK;
+}
+
+int Engine::expect_z(double* out)
+{
+    std::vector<double> diag(dim_);
+    int rc = get_diag(diag.data());
+    if (rc) return rc;
+    for (int q = 0; q < n_; q++) {
+        double s = 0;
+        for (uint64_t i = 0; i < dim_; i++) s += ((i >> q) & 1) ? -diag[i] : diag[i];
+        out[q] = s;
+    }
+    return DMX_OK;
+}
+
+int Engine::set_method(int m)
+{
+    if (m != DMX_METHOD_PLAIN && m != DMX_METHOD_PLAIN_LAUNCH && m != DMX_METHOD_FUSED) {
+        set_error("unknown method " + std::to_string(m));
+        return DMX_EINVAL;
+    }
+    if (uploaded_ && ((m == DMX_METHOD_FUSED) != (method_ == DMX_METHOD_FUSED))) {
+        // the uploaded table was built for the other executor
+        method_ = m;
+        uploaded_ = false;
+        return upload();
+    }
+    method_ = m;
+    return DMX_OK;
+}
+
+int Engine::set_fusion_max_k(uint32_t k)
+{
+    if (k < 1 || k > 5) {
+        set_error("fusion width must be in 1..5");
+        return DMX_EINVAL;
+    }
+    fusion_max_k_ = (int)k;
+    return DMX_OK;
+}
+
+std::string Engine::dump() const
+{
+    std::ostringstream ss;
+    for (const Gate& g : circuit_) ss << g.dump() << "\n";
+    return ss.str();
+}
+
+}  // namespace dmx

@@ -1,0 +1,28 @@
+// sweep.hpp -- host launchers for the plain-method kernels (sweep.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "prim.hpp"
+
+namespace dmx {
+
+// Blocks of 256 threads for the persistent loop: co-resident maximum on this
+// device, capped at one wave per 256 entries of a plane.
+hipError_t plain_persistent_grid(int n, int* blocks);
+
+// Pair primitives [first, last) of one pass (slow: row index) in one
+// cooperative launch.  4x4 primitives are not accepted here.
+hipError_t launch_plain_persistent(double* re, double* im, const PrimOp* d_ops, int first,
+                                   int last, int n, bool slow, int blocks, hipStream_t stream);
+
+// One primitive sweep; host_op is the host copy of *d_op (selects the kernel).
+hipError_t launch_plain_one(double* re, double* im, const PrimOp* d_op, const PrimOp& host_op,
+                            int n, bool slow, int max_blocks, hipStream_t stream);
+
+// In-place rho <- conj(rho^T).
+hipError_t launch_conj_transpose(double* re, double* im, int n, hipStream_t stream);
+
+// out[i] = re[i*dim + i].
+hipError_t launch_diag(const double* re, int n, double* out, hipStream_t stream);
+
+}  // namespace dmx

@@ -1,0 +1,197 @@
+"""ctypes binding of the C ABI (include/dmsim_amd.h) of libdmsim_amd.so.
+
+This is the reference-side binding a Python maintainer would add for the
+C-ABI library (the pybind11 module ``libdmsim_py_amdgpu_omp`` next to it is
+the drop-in for the reference's own module).  It raises if the library has
+not been built: there is no fallback path.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libdmsim_amd.so")
+
+DMX_OK, DMX_EINVAL, DMX_ESTATE, DMX_EHIP, DMX_ENOMEM = 0, -1, -2, -3, -4
+METHOD_PLAIN, METHOD_PLAIN_LAUNCH, METHOD_FUSED = 0, 1, 2
+
+# every symbol include/dmsim_amd.h declares
+EXPORTS = [
+    "dmx_create", "dmx_destroy", "dmx_append", "dmx_append_matrix", "dmx_upload", "dmx_run",
+    "dmx_clear_circuit", "dmx_reset", "dmx_reset_dm", "dmx_measure", "dmx_get_dm",
+    "dmx_get_diag", "dmx_expect_z", "dmx_info", "dmx_dump", "dmx_set_method",
+    "dmx_set_fusion_max_k", "dmx_last_run_stats", "dmx_last_error",
+]
+
+_lib = None
+
+
+class DmxError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"dmx error {code}: {msg}")
+        self.code = code
+
+
+def load():
+    """Load libdmsim_amd.so (built by ``make -C dm-sim_amd``)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"{LIB_PATH} is missing: build it with `make -C dm-sim_amd`")
+    lib = C.CDLL(LIB_PATH)
+    P = C.POINTER
+    vp = C.c_void_p
+    lib.dmx_create.argtypes = [C.c_uint32, C.c_uint32, P(vp)]
+    lib.dmx_destroy.argtypes = [vp]
+    lib.dmx_append.argtypes = [vp, C.c_int32, P(C.c_uint32), C.c_double, C.c_double, C.c_double]
+    lib.dmx_append_matrix.argtypes = [vp, C.c_uint32, P(C.c_uint32), P(C.c_double),
+                                      P(C.c_double)]
+    for name in ("dmx_upload", "dmx_run", "dmx_clear_circuit", "dmx_reset", "dmx_reset_dm"):
+        getattr(lib, name).argtypes = [vp]
+    lib.dmx_measure.argtypes = [vp, C.c_uint32, C.c_uint32, P(C.c_uint32), P(C.c_double)]
+    lib.dmx_get_dm.argtypes = [vp, P(C.c_double), P(C.c_double)]
+    lib.dmx_get_diag.argtypes = [vp, P(C.c_double)]
+    lib.dmx_expect_z.argtypes = [vp, P(C.c_double)]
+    lib.dmx_info.argtypes = [vp, P(C.c_uint32), P(C.c_uint64), P(C.c_uint32)]
+    lib.dmx_dump.argtypes = [vp, C.c_char_p, C.c_size_t, P(C.c_size_t)]
+    lib.dmx_set_method.argtypes = [vp, C.c_int]
+    lib.dmx_set_fusion_max_k.argtypes = [vp, C.c_uint32]
+    lib.dmx_last_run_stats.argtypes = [vp, P(C.c_double), P(C.c_uint32), P(C.c_uint32)]
+    lib.dmx_last_error.restype = C.c_char_p
+    for name in EXPORTS:
+        if name != "dmx_last_error":
+            getattr(lib, name).restype = C.c_int
+    _lib = lib
+    return lib
+
+
+def _ptr(a, ctype):
+    return a.ctypes.data_as(C.POINTER(ctype))
+
+
+def _check(rc):
+    if rc != DMX_OK:
+        msg = load().dmx_last_error().decode()
+        if rc == DMX_EINVAL:
+            raise ValueError(msg)
+        raise DmxError(rc, msg)
+
+
+class Sim:
+    """One rho on the current GPU (the reference's ``Simulation``)."""
+
+    def __init__(self, n_qubits: int, n_gpus: int = 1):
+        lib = load()
+        h = C.c_void_p()
+        _check(lib.dmx_create(n_qubits, n_gpus, C.byref(h)))
+        self._h = h
+        self.n = n_qubits
+        self.dim = 1 << n_qubits
+
+    def close(self):
+        if getattr(self, "_h", None):
+            load().dmx_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def append(self, op: int, qubits=(), theta=0.0, phi=0.0, lam=0.0):
+        qb = np.zeros(5, dtype=np.uint32)
+        qb[: len(qubits)] = qubits
+        _check(load().dmx_append(self._h, int(op), _ptr(qb, C.c_uint32), theta, phi, lam))
+
+    def append_matrix(self, qubits, mat):
+        mat = np.asarray(mat, dtype=complex)
+        re = np.ascontiguousarray(mat.real, dtype=np.float64)
+        im = np.ascontiguousarray(mat.imag, dtype=np.float64)
+        q = np.asarray(qubits, dtype=np.uint32)
+        _check(load().dmx_append_matrix(self._h, len(q), _ptr(q, C.c_uint32),
+                                        _ptr(re, C.c_double), _ptr(im, C.c_double)))
+
+    def append_table(self, table):
+        """Append every gate of a workloads.GateTable."""
+        ops, qb, prm, coef = table.arrays()
+        for g in range(len(ops)):
+            op = int(ops[g])
+            if op >= 38:  # matrix extensions C1 (38) / C2 (39)
+                k = op - 37
+                d = 1 << k
+                flat = coef[g, : 2 * d * d]
+                self.append_matrix(list(qb[g, :k]), (flat[0::2] + 1j * flat[1::2]).reshape(d, d))
+            else:
+                self.append(op, list(qb[g]), *prm[g])
+
+    def upload(self):
+        _check(load().dmx_upload(self._h))
+        return self
+
+    def run(self):
+        _check(load().dmx_run(self._h))
+
+    def clear_circuit(self):
+        _check(load().dmx_clear_circuit(self._h))
+
+    def reset(self):
+        _check(load().dmx_reset(self._h))
+
+    def reset_dm(self):
+        _check(load().dmx_reset_dm(self._h))
+
+    def measure(self, reps: int, seed: int):
+        out = np.zeros(max(reps, 1), dtype=np.uint32)
+        tot = C.c_double()
+        _check(load().dmx_measure(self._h, reps, seed, _ptr(out, C.c_uint32), C.byref(tot)))
+        return out[:reps], tot.value
+
+    def get_dm(self):
+        re = np.empty(self.dim * self.dim)
+        im = np.empty(self.dim * self.dim)
+        _check(load().dmx_get_dm(self._h, _ptr(re, C.c_double), _ptr(im, C.c_double)))
+        return re, im
+
+    def get_diag(self):
+        d = np.empty(self.dim)
+        _check(load().dmx_get_diag(self._h, _ptr(d, C.c_double)))
+        return d
+
+    def expect_z(self):
+        z = np.empty(self.n)
+        _check(load().dmx_expect_z(self._h, _ptr(z, C.c_double)))
+        return z
+
+    def info(self):
+        n, dim, g = C.c_uint32(), C.c_uint64(), C.c_uint32()
+        _check(load().dmx_info(self._h, C.byref(n), C.byref(dim), C.byref(g)))
+        return n.value, dim.value, g.value
+
+    def dump(self):
+        need = C.c_size_t()
+        _check(load().dmx_dump(self._h, None, 0, C.byref(need)))
+        buf = C.create_string_buffer(need.value)
+        _check(load().dmx_dump(self._h, buf, need.value, None))
+        return buf.value.decode()
+
+    def set_method(self, m: int):
+        _check(load().dmx_set_method(self._h, m))
+
+    def set_fusion_max_k(self, k: int):
+        _check(load().dmx_set_fusion_max_k(self._h, k))
+
+    def last_run_stats(self):
+        ms, la, sw = C.c_double(), C.c_uint32(), C.c_uint32()
+        _check(load().dmx_last_run_stats(self._h, C.byref(ms), C.byref(la), C.byref(sw)))
+        return {"ms": ms.value, "launches": la.value, "sweeps": sw.value}

@@ -1,0 +1,141 @@
+/*
+ * dmsim_amd.h -- C ABI of the MI355X density-matrix engine (libdmsim_amd.so).
+ *
+ * Drop-in boundary for the single-GPU path of pnnl/DM-Sim: every entry point
+ * below replaces one member of the reference's `DMSim::Simulation`
+ * (src/dmsim_amdgpu_omp.hpp) or of its pybind11 module
+ * (src/py_amdgpu_omp_wrapper.cpp:29-88); the comment on each names the
+ * reference interface (file:line) it stands for.  Plain pointers and sizes
+ * only.  Every function returns DMX_OK (0) or a negative DMX_E* code; the
+ * message of the last failure on the calling thread is dmx_last_error().
+ * The reference asserts or exit()s in the same situations.
+ *
+ * State convention (identical to the reference's dm_real_res/dm_imag_res):
+ * dmx_get_dm() returns re[i*dim+j] = Re(rho_ji), im[i*dim+j] = Im(rho_ji),
+ * i.e. conj(rho) in row-major order, dim = 2^n_qubits; qubit q is bit q of a
+ * basis index.
+ *
+ * A handle is not thread-safe (neither is the reference's object).
+ */
+#ifndef DMSIM_AMD_H
+#define DMSIM_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define DMX_OK 0
+#define DMX_EINVAL (-1)  /* bad argument: qubit out of range, n_gpus != 1, ... */
+#define DMX_ESTATE (-2)  /* call out of order: run() before upload(), ...      */
+#define DMX_EHIP (-3)    /* HIP runtime error                                   */
+#define DMX_ENOMEM (-4)  /* device or host allocation failed                    */
+
+/* Op codes: the reference's `enum OP` in order (src/dmsim_amdgpu_omp.hpp:51-57). */
+enum dmx_op {
+    DMX_U3 = 0, DMX_U2, DMX_U1, DMX_CX, DMX_ID, DMX_X, DMX_Y, DMX_Z, DMX_H, DMX_S,
+    DMX_SDG, DMX_T, DMX_TDG, DMX_RX, DMX_RY, DMX_RZ, DMX_CZ, DMX_CY, DMX_SWAP, DMX_CH,
+    DMX_CCX, DMX_CSWAP, DMX_CRX, DMX_CRY, DMX_CRZ, DMX_CU1, DMX_CU3, DMX_RXX, DMX_RZZ,
+    DMX_RCCX, DMX_RC3X, DMX_C3X, DMX_C3SQRTX, DMX_C4X, DMX_R, DMX_SRN, DMX_W, DMX_RYY,
+    DMX_OP_COUNT
+};
+
+/* Execution methods (dmx_set_method). */
+#define DMX_METHOD_PLAIN 0        /* persistent cooperative gate loop, 2 sweeps per primitive */
+#define DMX_METHOD_PLAIN_LAUNCH 1 /* same sweeps, one launch per primitive                    */
+#define DMX_METHOD_FUSED 2        /* gates fused into dense k<=5 blocks, one sweep per block  */
+
+typedef struct dmx_sim dmx_sim;
+
+/* Simulation::Simulation(n_qubits, n_gpus), src/dmsim_amdgpu_omp.hpp:211-286.
+ * Allocates rho on the current HIP device and sets it to |0..0><0..0|.
+ * n_gpus must be 1 (one GPU per process); 1 <= n_qubits <= 16. */
+int dmx_create(uint32_t n_qubits, uint32_t n_gpus, dmx_sim** out);
+
+/* Simulation::~Simulation, src/dmsim_amdgpu_omp.hpp:288-316. NULL is a no-op. */
+int dmx_destroy(dmx_sim* sim);
+
+/* Simulation::append(Gate*), src/dmsim_amdgpu_omp.hpp:346-359, with the Gate
+ * record of src/dmsim_amdgpu_omp.hpp:111-118 passed by value: op, qb[5]
+ * (operands first; unused slots ignored), theta, phi, lambda as the static
+ * factories place them (:594-782).  Operands must be < n_qubits and
+ * distinct. */
+int dmx_append(dmx_sim* sim, int32_t op, const uint32_t qb[5], double theta, double phi,
+               double lambda);
+
+/* Extension (no OP reaches it in the reference): an arbitrary k-qubit matrix,
+ * k = 1..5, row-major 2^k x 2^k split planes, qubits[0] the most significant
+ * bit of the matrix index.  k = 1 and k = 2 are the reference's C1_GATE and
+ * C2_GATE (src/dmsim_amdgpu_omp.hpp:1019-1137; C2's qubit1 is qubits[0]).
+ * k >= 3 runs only under DMX_METHOD_FUSED. */
+int dmx_append_matrix(dmx_sim* sim, uint32_t k, const uint32_t* qubits, const double* mat_re,
+                      const double* mat_im);
+
+/* Simulation::upload(), src/dmsim_amdgpu_omp.hpp:360-392: one packed copy of
+ * the lowered gate table to the device.  A second upload() without
+ * clear_circuit() in between is DMX_ESTATE (the reference asserts). */
+int dmx_upload(dmx_sim* sim);
+
+/* Simulation::sim() (Python: run), src/dmsim_amdgpu_omp.hpp:405-509: applies
+ * the uploaded circuit to the current state, rho <- U rho U^dagger. */
+int dmx_run(dmx_sim* sim);
+
+/* Simulation::clear_circuit(), src/dmsim_amdgpu_omp.hpp:511-535: drops the
+ * gates, keeps rho (the next run continues from it). */
+int dmx_clear_circuit(dmx_sim* sim);
+
+/* Simulation::reset(), src/dmsim_amdgpu_omp.hpp:318-322: clear_circuit() +
+ * reset_dm(). */
+int dmx_reset(dmx_sim* sim);
+
+/* Simulation::reset_dm(), src/dmsim_amdgpu_omp.hpp:323-344: rho <- |0><0|. */
+int dmx_reset_dm(dmx_sim* sim);
+
+/* Simulation::measure(repetition), src/dmsim_amdgpu_omp.hpp:536-564: samples
+ * `repetition` basis states from |Re rho_ii| with glibc srand(seed)/rand()
+ * (the reference seeds with time(0), src/config.hpp:47); out[repetition].
+ * *prob_sum (may be NULL) receives the sum of |Re rho_ii|. */
+int dmx_measure(dmx_sim* sim, uint32_t repetition, uint32_t seed, uint32_t* out,
+                double* prob_sum);
+
+/* The public result arrays dm_real_res / dm_imag_res
+ * (src/dmsim_amdgpu_omp.hpp:473-474), copied on demand into caller buffers of
+ * 4^n doubles each.  Either pointer may be NULL. */
+int dmx_get_dm(dmx_sim* sim, double* re, double* im);
+
+/* Re rho_ii, i < 2^n (the diagonal measure() and print_res_sv() read,
+ * src/dmsim_amdgpu_omp.hpp:541-543, :565-575). */
+int dmx_get_diag(dmx_sim* sim, double* diag);
+
+/* Extension: <Z_q> = sum_i (-1)^{bit q of i} Re rho_ii for q < n; out[n]. */
+int dmx_expect_z(dmx_sim* sim, double* out);
+
+/* Public members n_qubits, dim, n_gates (src/dmsim_amdgpu_omp.hpp:786-800). */
+int dmx_info(const dmx_sim* sim, uint32_t* n_qubits, uint64_t* dim, uint32_t* n_gates);
+
+/* Simulation::dump(), src/dmsim_amdgpu_omp.hpp:394-402: the circuit as text,
+ * one "OP(qb0,qb1,qb2,qb3,qb4,theta,phi,lambda);" line per gate.  Writes at
+ * most cap bytes including the terminating NUL; *needed gets the full size. */
+int dmx_dump(const dmx_sim* sim, char* buf, size_t cap, size_t* needed);
+
+/* Execution method for subsequent run() calls (default DMX_METHOD_PLAIN). */
+int dmx_set_method(dmx_sim* sim, int method);
+
+/* Fusion limit for DMX_METHOD_FUSED: largest block width k (1..5). */
+int dmx_set_fusion_max_k(dmx_sim* sim, uint32_t k);
+
+/* Device time of the last run() in milliseconds (HIP events on the run's
+ * stream), number of kernel launches it made, and number of full-matrix
+ * sweeps it performed. */
+int dmx_last_run_stats(const dmx_sim* sim, double* ms, uint32_t* launches, uint32_t* sweeps);
+
+/* Message of the last failure on this thread ("" if none). */
+const char* dmx_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* DMSIM_AMD_H */

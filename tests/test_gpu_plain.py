@@ -1,0 +1,246 @@
+"""GPU parity of the plain method through the C ABI (libdmsim_amd.so).
+
+The plain sweeps evaluate the reference's per-element arithmetic without FMA
+contraction, so rho must equal the oracle (and thereby the reference CPU path)
+bit for bit; shots under a shared seed must be identical.  Large n is checked
+against the state-vector oracle through size-independent properties.
+"""
+import hashlib
+import math
+import os
+
+import numpy as np
+import pytest
+
+import workloads as W
+from conftest import GOLDEN
+
+pytestmark = pytest.mark.gpu
+
+METHODS = [0, 1]  # persistent gate loop, one launch per primitive
+
+
+def dmx():
+    import dmsim_amd
+    return dmsim_amd
+
+
+def run_engine(n, tables, method=0):
+    """Runs each table as one segment (clear_circuit / append / upload / run)."""
+    s = dmx().Sim(n)
+    s.set_method(method)
+    for t in tables:
+        s.clear_circuit()
+        s.append_table(t)
+        s.upload()
+        s.run()
+    return s
+
+
+def table_from(d, prefix=""):
+    t = W.GateTable(int(0))
+    ops, qb, prm, coef = d[prefix + "ops"], d[prefix + "qb"], d[prefix + "prm"], d[prefix + "coef"]
+    t.ops, t.qbs, t.prms, t.coefs = list(ops), [list(r) for r in qb], [list(r) for r in prm], \
+        [np.array(r) for r in coef]
+    return t
+
+
+def sha(a):
+    return hashlib.sha256(np.ascontiguousarray(a, dtype="<f8").tobytes()).hexdigest()
+
+
+@pytest.mark.parametrize("method", METHODS)
+@pytest.mark.parametrize("n", [1, 2, 3, 4, 5, 6])
+def test_all_ops_small_n_bitwise(oracle, n, method):
+    gate_set = [g for g in W.OP_NAMES if W.ARITY[g] <= n]
+    for seed in range(2):
+        t = W.random_mixed(n, 150, 500 + 7 * n + seed, gate_set=gate_set)
+        s = run_engine(n, [t], method)
+        re, im = s.get_dm()
+        ore, oim = oracle.sim(n, t.arrays())
+        assert np.array_equal(re, ore) and np.array_equal(im, oim), (n, seed)
+
+
+@pytest.mark.parametrize("method", METHODS)
+def test_golden_circuits_n6(method):
+    d = np.load(os.path.join(GOLDEN, "circuits_n6.npz"))
+    for c in range(4):
+        s = run_engine(6, [table_from(d, f"c{c}_")], method)
+        re, im = s.get_dm()
+        assert np.array_equal(re, d[f"c{c}_re"]) and np.array_equal(im, d[f"c{c}_im"]), c
+
+
+@pytest.mark.parametrize("method", METHODS)
+def test_golden_mixed_n8_and_shots(method):
+    d = np.load(os.path.join(GOLDEN, "mixed_n8.npz"))
+    for seed in range(1, 9):
+        s = run_engine(8, [table_from(d, f"s{seed}_")], method)
+        shots, total = s.measure(64, int(d[f"s{seed}_shot_seed"]))
+        re, im = s.get_dm()
+        assert sha(re) == str(d[f"s{seed}_sha_re"]) and sha(im) == str(d[f"s{seed}_sha_im"])
+        assert np.array_equal(shots, d[f"s{seed}_shots"]), seed
+        assert abs(total - 1) < 1e-12
+        z = s.expect_z()
+        assert np.allclose(z, d[f"s{seed}_zexp"], atol=1e-12)
+
+
+def test_adder_two_segments_like_the_python_example():
+    """example/adder_n10_omp.py: X gates, run, clear_circuit, adder, run."""
+    first, second = W.adder_n10()
+    s = run_engine(10, [first, second])
+    shots, _ = s.measure(10, 7)
+    assert list(shots) == [514] * 10
+
+
+@pytest.mark.parametrize("method", METHODS)
+def test_continuation_bitwise(oracle, method):
+    """Three segments on one state: orientation flips every run; every
+    intermediate and final rho equals the oracle's chained runs bit for bit."""
+    n = 7
+    segs = [W.random_mixed(n, 80, 70 + i) for i in range(3)]
+    s = dmx().Sim(n)
+    s.set_method(method)
+    state = None
+    for t in segs:
+        s.clear_circuit()
+        s.append_table(t)
+        s.upload()
+        s.run()
+        state = oracle.sim(n, t.arrays(), state)
+        re, im = s.get_dm()
+        assert np.array_equal(re, state[0]) and np.array_equal(im, state[1])
+
+
+def test_continuation_without_download(oracle):
+    """Runs back to back with no get_dm in between (the state stays
+    transposed on the device between runs)."""
+    n = 6
+    segs = [W.random_mixed(n, 50, 90 + i) for i in range(4)]
+    s = dmx().Sim(n)
+    state = None
+    for t in segs:
+        s.clear_circuit()
+        s.append_table(t)
+        s.upload()
+        s.run()
+        state = oracle.sim(n, t.arrays(), state)
+    re, im = s.get_dm()
+    assert np.array_equal(re, state[0]) and np.array_equal(im, state[1])
+
+
+@pytest.mark.parametrize("n,gates", [(9, 200), (10, 120), (11, 60)])
+def test_medium_n_bitwise(oracle, n, gates):
+    oracle.set_threads(min(os.cpu_count() or 1, 16))
+    t = W.random_mixed(n, gates, 1000 + n)
+    for method in METHODS:
+        s = run_engine(n, [t], method)
+        re, im = s.get_dm()
+        ore, oim = oracle.sim(n, t.arrays())
+        assert np.array_equal(re, ore) and np.array_equal(im, oim), method
+
+
+def test_brickwall_n12_bitwise(oracle):
+    oracle.set_threads(min(os.cpu_count() or 1, 16))
+    t = W.brickwall_su4(n=12, layers=4, tail=40, seed=12)
+    s = run_engine(12, [t], 1)
+    re, im = s.get_dm()
+    ore, oim = oracle.sim(12, t.arrays())
+    assert np.array_equal(re, ore) and np.array_equal(im, oim)
+
+
+def test_n14_brickwall_against_state_vector(oracle):
+    """The n = 14 SU(4) brick-wall + U3/CX workload: diagonal, trace, sampled
+    entries and Hermiticity against the pure-state oracle; shots bit-exact
+    against the oracle's sampler on the engine's own diagonal."""
+    n = 14
+    t = W.brickwall_su4(n=n, layers=20, tail=2000, seed=14)
+    s = run_engine(n, [t], 1)
+    psi = oracle.sv(n, t.arrays())
+    diag = s.get_diag()
+    assert np.max(np.abs(diag - np.abs(psi) ** 2)) < 1e-12
+    assert abs(diag.sum() - 1) < 1e-12
+    shots, _ = s.measure(200, 99)
+    want, _ = oracle.measure_diag(n, diag, 200, 99)
+    assert np.array_equal(shots, want)
+    re, im = s.get_dm()
+    dim = 1 << n
+    rng = np.random.default_rng(0)
+    i = rng.integers(0, dim, 4096)
+    j = rng.integers(0, dim, 4096)
+    got = re[i * dim + j] + 1j * im[i * dim + j]
+    want = np.conj(psi[i]) * psi[j]
+    assert np.max(np.abs(got - want)) < 1e-10
+    herm = re[j * dim + i] - 1j * im[j * dim + i]
+    assert np.max(np.abs(got - herm)) < 1e-12
+
+
+def test_errors_follow_reference_asserts():
+    d = dmx()
+    s = d.Sim(4)
+    with pytest.raises(d.DmxError):
+        s.run()  # before upload()
+    with pytest.raises(ValueError):
+        s.append(W.OP["H"], [4])
+    with pytest.raises(ValueError):
+        s.append(W.OP["CX"], [1, 1])
+    s.append(W.OP["H"], [0])
+    s.upload()
+    with pytest.raises(d.DmxError):
+        s.upload()  # twice without clear_circuit()
+    s.run()
+    assert "H(0,0,0,0,0,0,0,0);" in s.dump()
+    assert s.info() == (4, 16, 1)
+
+
+def test_pybind_module_runs_the_reference_example():
+    """The reference's Python example, verbatim API, through the drop-in module."""
+    import libdmsim_py_amdgpu_omp as dmsim_omp
+    sim = dmsim_omp.Simulation(10, 1)
+    for q in (1, 5, 6, 7, 8):
+        sim.append(sim.X(q))
+    sim.upload()
+    sim.run()
+    sim.clear_circuit()
+
+    def majority(a, b, c):
+        sim.append(sim.CX(c, b))
+        sim.append(sim.CX(c, a))
+        sim.append(sim.CCX(a, b, c))
+
+    def unmaj(a, b, c):
+        sim.append(sim.CCX(a, b, c))
+        sim.append(sim.CX(c, a))
+        sim.append(sim.CX(a, b))
+
+    majority(0, 5, 1)
+    majority(1, 6, 2)
+    majority(2, 7, 3)
+    majority(3, 8, 4)
+    sim.append(sim.CX(4, 9))
+    unmaj(3, 8, 4)
+    unmaj(2, 7, 3)
+    unmaj(1, 6, 2)
+    unmaj(0, 5, 1)
+    sim.upload()
+    sim.run()
+    res = sim.measure(10)
+    assert res == [514] * 10
+    assert abs(sim.dm_real_res[514, 514] - 1.0) < 1e-12
+
+
+def test_rx_and_deuteron_on_gpu():
+    import libdmsim_py_amdgpu_omp as m
+    for th in np.linspace(-math.pi, math.pi, 7):
+        sim = m.Simulation(1, 1)
+        sim.append(sim.RX(th, 0))
+        sim.upload()
+        sim.run()
+        assert abs(sim.expect_z()[0] - math.cos(th)) < 1e-14
+    for tval in np.linspace(-math.pi, math.pi, 20):
+        sim = m.Simulation(2, 1)
+        for g in (sim.X(0), sim.RY(tval, 1), sim.CX(1, 0), sim.H(0), sim.H(1)):
+            sim.append(g)
+        sim.upload()
+        sim.run()
+        d = sim.get_diag()
+        assert abs((d[0] - d[1] - d[2] + d[3]) - math.sin(tval)) < 1e-13
