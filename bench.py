@@ -73,7 +73,7 @@ def main():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--n", type=int, default=14)
-    ap.add_argument("--methods", default="plain_launch,plain")
+    ap.add_argument("--methods", default="fused,plain_launch")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:

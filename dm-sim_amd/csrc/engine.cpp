@@ -139,10 +139,16 @@ int Engine::upload()
                               hipMemcpyHostToDevice, stream_));
     }
     if (method_ == DMX_METHOD_FUSED) {
+        // no plan (SRN in the circuit, or n < 4): the run uses the plain sweeps
         int rc = build_fused_plan(circuit_, n_, fusion_max_k_, &plan_);
         if (rc) return rc;
-        rc = plan_->upload(stream_);
-        if (rc) return rc;
+        if (plan_) {
+            rc = plan_->upload(stream_);
+            if (rc) return rc;
+        } else if (wide) {
+            set_error("matrix gates on 3..5 qubits need a fusable circuit (n >= 4, no SRN)");
+            return DMX_ESTATE;
+        }
     }
     HIPTRY(hipStreamSynchronize(stream_));
     uploaded_ = true;
@@ -155,14 +161,8 @@ int Engine::run()
         set_error("run() before upload()");
         return DMX_ESTATE;
     }
-    if (method_ == DMX_METHOD_FUSED) {
-        if (!plan_) {
-            set_error("circuit was uploaded under another method; clear_circuit() and upload()");
-            return DMX_ESTATE;
-        }
-        return run_fused();
-    }
-    return run_plain(method_ == DMX_METHOD_PLAIN_LAUNCH);
+    if (method_ == DMX_METHOD_FUSED && plan_) return run_fused();
+    return run_plain(method_ != DMX_METHOD_PLAIN);
 }
 
 int Engine::run_plain(bool per_launch)

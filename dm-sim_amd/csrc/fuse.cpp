@@ -1,16 +1,246 @@
-// fuse.cpp -- fusion planner and fused executor.
+// fuse.cpp -- fusion planner and fused executor (host side).
+//
+// Planner: gates are taken in circuit order; a gate joins an existing block
+// when the union of qubits stays within the width limit and no later block
+// touches its qubits (so moving it there only reorders gates on disjoint
+// qubits, which commute).  Among the admissible blocks the one that grows the
+// least is chosen.  Every block is then padded to four qubits with the lowest
+// qubits it does not use (identity there) and its 16 x 16 matrix is the
+// product of the member gates' matrices, each composed from the gate's own
+// primitive sweeps (so C4X, C3SQRTX, R keep the reference's semantics).
 #include "fuse.hpp"
+
+#include <algorithm>
+#include <complex>
+#include <cstring>
 
 #include "dmsim_amd.h"
 #include "engine.hpp"
+#include "fused.hpp"
+#include "sweep.hpp"
 
 namespace dmx {
 
-int build_fused_plan(const std::vector<Gate>&, int, int, FusedPlan** out)
+namespace {
+
+using cd = std::complex<double>;
+
+// A run is a sequence of steps: a fused block (one sweep), or a plain segment
+// of primitive sweeps for a gate wider than the block width, applied to the
+// fast index and then to the slow index (valid for complex-linear gates, whose
+// row and column actions commute).
+struct Step {
+    bool fused;
+    int index;  // block index, or first primitive of the segment
+    int count;  // primitives in the segment
+};
+
+struct Plan : FusedPlan {
+    int n = 0;
+    std::vector<FusedBlock> blocks;
+    std::vector<uint32_t> gates_per_block;
+    std::vector<PrimOp> prims;
+    std::vector<Step> steps;
+    FusedBlock* d_blocks = nullptr;
+    PrimOp* d_prims = nullptr;
+    int max_blocks = 4096;
+
+    ~Plan() override
+    {
+        if (d_blocks) (void)hipFree(d_blocks);
+        if (d_prims) (void)hipFree(d_prims);
+    }
+
+    template <class T>
+    static int put(T** dst, const std::vector<T>& src, hipStream_t stream)
+    {
+        if (src.empty()) return DMX_OK;
+        hipError_t e = hipMalloc(dst, src.size() * sizeof(T));
+        if (e != hipSuccess) {
+            set_error(std::string("hipMalloc(fused plan): ") + hipGetErrorString(e));
+            return e == hipErrorOutOfMemory ? DMX_ENOMEM : DMX_EHIP;
+        }
+        e = hipMemcpyAsync(*dst, src.data(), src.size() * sizeof(T), hipMemcpyHostToDevice,
+                           stream);
+        if (e != hipSuccess) {
+            set_error(std::string("hipMemcpyAsync(fused plan): ") + hipGetErrorString(e));
+            return DMX_EHIP;
+        }
+        return DMX_OK;
+    }
+
+    int upload(hipStream_t stream) override
+    {
+        int rc = put(&d_blocks, blocks, stream);
+        return rc ? rc : put(&d_prims, prims, stream);
+    }
+
+    int execute(double* re, double* im, int nq, bool&, PrimOp*, hipStream_t stream, int,
+                uint32_t* launches, uint32_t* sweeps) override
+    {
+        uint32_t la = 0;
+        for (const Step& st : steps) {
+            hipError_t e = hipSuccess;
+            if (st.fused) {
+                e = launch_fused4(re, im, d_blocks + st.index, nq, max_blocks, stream);
+                la++;
+            } else {
+                for (int side = 0; side < 2 && e == hipSuccess; side++)
+                    for (int t = st.index; t < st.index + st.count && e == hipSuccess; t++) {
+                        e = launch_plain_one(re, im, d_prims + t, prims[t], nq, side == 1, 8192,
+                                             stream);
+                        la++;
+                    }
+            }
+            if (e != hipSuccess) {
+                set_error(std::string("fused run launch: ") + hipGetErrorString(e));
+                return DMX_EHIP;
+            }
+        }
+        *launches = la;
+        *sweeps = la;
+        return DMX_OK;
+    }
+};
+
+uint32_t gate_mask(const Gate& g)
+{
+    const int k = g.op == OP_MAT ? g.k : op_arity(g.op);
+    uint32_t m = 0;
+    for (int i = 0; i < k; i++) m |= 1u << g.qb[i];
+    return m;
+}
+
+// Left-multiplies the 16x16 block matrix U by gate G embedded on the block's
+// local bits.  G's index bit (kg-1-j) belongs to qubit gq[j].
+void apply_embedded(std::vector<cd>& U, uint32_t qmask, const std::vector<uint32_t>& gq,
+                    const std::vector<double>& gre, const std::vector<double>& gim)
+{
+    int local[32];
+    {
+        int t = 0;
+        for (int q = 0; q < 32; q++)
+            if (qmask & (1u << q)) local[q] = t++;
+    }
+    const int kg = (int)gq.size();
+    const int dg = 1 << kg;
+    uint32_t gl = 0;  // gate's local bit mask
+    for (int jj = 0; jj < kg; jj++) gl |= 1u << local[gq[jj]];
+    auto sub = [&](int x) {
+        int s = 0;
+        for (int jj = 0; jj < kg; jj++)
+            if (x & (1 << local[gq[jj]])) s |= 1 << (kg - 1 - jj);
+        return s;
+    };
+    std::vector<cd> out(256, 0.0);
+    for (int xo = 0; xo < 16; xo++) {
+        const int so = sub(xo);
+        for (int xi = 0; xi < 16; xi++) {
+            if ((xo & ~gl) != (xi & ~gl)) continue;
+            const int si = sub(xi);
+            const cd gv(gre[(size_t)so * dg + si], gim[(size_t)so * dg + si]);
+            if (gv == 0.0) continue;
+            for (int c = 0; c < 16; c++) out[(size_t)xo * 16 + c] += gv * U[(size_t)xi * 16 + c];
+        }
+    }
+    U.swap(out);
+}
+
+}  // namespace
+
+int build_fused_plan(const std::vector<Gate>& circuit, int n, int max_k, FusedPlan** out)
 {
     *out = nullptr;
-    set_error("fused method not available in this build");
-    return DMX_ESTATE;
+    if (n < 4) return DMX_OK;  // too few qubits for a 4-qubit block: plain sweeps
+    const int kmax = std::min(std::max(max_k, 1), 4);
+    for (const Gate& g : circuit) {
+        if (g.op == OP_SRN) return DMX_OK;  // not complex-linear: plain sweeps
+        if (g.op == OP_MAT && g.k > 4) {
+            set_error("the fused executor takes matrix gates on at most 4 qubits");
+            return DMX_EINVAL;
+        }
+    }
+    // planning: blocks[frozen..] may still take gates; a wide gate freezes all
+    struct PB {
+        uint32_t mask;
+        std::vector<int> gates;
+    };
+    std::vector<PB> pbs;
+    std::vector<std::pair<int, int>> order;  // (kind 0 = block / 1 = wide gate, index)
+    int frozen = 0;
+    for (int gi = 0; gi < (int)circuit.size(); gi++) {
+        const Gate& g = circuit[gi];
+        if (g.op == OP_ID) continue;
+        const uint32_t S = gate_mask(g);
+        const int k_here = __builtin_popcount(S);
+        if (k_here > 4) {
+            order.push_back({1, gi});
+            frozen = (int)pbs.size();
+            continue;
+        }
+        int jstar = frozen - 1;
+        for (int b = (int)pbs.size() - 1; b >= frozen; b--)
+            if (pbs[b].mask & S) {
+                jstar = b;
+                break;
+            }
+        int best = -1, best_w = 99;
+        for (int b = std::max(jstar, frozen); b < (int)pbs.size(); b++) {
+            const int w = __builtin_popcount(pbs[b].mask | S);
+            if (w <= std::max(kmax, k_here) && w <= best_w) {
+                best = b;
+                best_w = w;
+            }
+        }
+        if (best >= 0) {
+            pbs[best].mask |= S;
+            pbs[best].gates.push_back(gi);
+        } else {
+            order.push_back({0, (int)pbs.size()});
+            pbs.push_back({S, {gi}});
+        }
+    }
+
+    Plan* plan = new Plan();
+    plan->n = n;
+    const uint32_t all = (1u << n) - 1;
+    for (const PB& pb : pbs) {
+        uint32_t qmask = pb.mask;
+        for (int q = 0; q < n && __builtin_popcount(qmask) < 4; q++) qmask |= 1u << q;
+        uint32_t cmask = qmask;
+        const int m = std::min(8, n);
+        for (int q = 0; q < n && __builtin_popcount(cmask) < m; q++) cmask |= 1u << q;
+        std::vector<cd> U(256, 0.0);
+        for (int i = 0; i < 16; i++) U[(size_t)i * 16 + i] = 1.0;
+        for (int gi : pb.gates) {
+            int k = 0;
+            std::vector<uint32_t> gq;
+            std::vector<double> gre, gim;
+            gate_matrix(circuit[gi], k, gq, gre, gim);
+            apply_embedded(U, qmask, gq, gre, gim);
+        }
+        FusedBlock fb;
+        std::memset(&fb, 0, sizeof(fb));
+        fb.qmask = qmask & all;
+        fb.cmask = cmask & all;
+        for (int i = 0; i < 256; i++) {
+            fb.ur[i] = U[i].real();
+            fb.ui[i] = U[i].imag();
+        }
+        plan->blocks.push_back(fb);
+        plan->gates_per_block.push_back((uint32_t)pb.gates.size());
+    }
+    for (const auto& o : order) {
+        if (o.first == 0) {
+            plan->steps.push_back({true, o.second, 0});
+        } else {
+            const int first = (int)plan->prims.size();
+            lower_gate(circuit[o.second], plan->prims);
+            plan->steps.push_back({false, first, (int)plan->prims.size() - first});
+        }
+    }
+    *out = plan;
+    return DMX_OK;
 }
 
 }  // namespace dmx

@@ -1,0 +1,181 @@
+// fused.hip -- one sweep of rho applies a dense 4-qubit block on both sides,
+// rho <- U rho U^dagger, i.e. A <- conj(U) A U^T for the stored A = conj(rho),
+// with FP64 MFMA (v_mfma_f64_16x16x4f64) on gfx950.
+//
+// Tile: the 16 x 16 entries whose row and column indices range over the
+// block's four qubits Q (other bits fixed).  A workgroup stages a super-tile
+// in LDS: those 16 rows x 256 columns, the columns spanning Q plus the lowest
+// free column bits, so every global access is a contiguous run of at least 16
+// doubles (the four lowest column bits are always inside the super-tile).
+// Each of the 16 tiles of a super-tile is transformed by one wave:
+//   X  = T U^T      (16 MFMAs, complex as four real products)
+//   T' = conj(U) X  (16 MFMAs)
+// MFMA lane layout on gfx950 (measured): A[i][k] in lane i+16k, B[k][j] in
+// lane j+16k, D[4r+g][j] in lane j+16g register r -- so register c of X is
+// exactly the B operand of the second product's K-chunk c, and the block
+// coefficients U[j][4c+k] held in lane j+16k serve as B (U^T) in the first
+// product and as A (conj(U)) in the second.
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "fused.hpp"
+
+namespace dmx {
+
+namespace {
+
+typedef double d4 __attribute__((ext_vector_type(4)));
+
+constexpr int kPitch = 257;  // LDS row pitch in doubles (odd: spreads rows over banks)
+
+__device__ __forceinline__ d4 mfma(double a, double b, d4 c)
+{
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+
+// Scatter the low bits of v into the positions of `mask` (ascending).
+__device__ __forceinline__ uint64_t deposit(uint64_t v, uint64_t mask)
+{
+    uint64_t out = 0;
+    for (uint64_t m = mask; m; m &= m - 1) {
+        const uint64_t bit = m & (~m + 1);
+        if (v & 1) out |= bit;
+        v >>= 1;
+    }
+    return out;
+}
+
+}  // namespace
+
+__global__ void __launch_bounds__(256, 2) k_fused4(double* __restrict__ re, double* __restrict__ im,
+                                                    const FusedBlock* __restrict__ blk, int n)
+{
+    __shared__ double s_re[16 * kPitch];
+    __shared__ double s_im[16 * kPitch];
+
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const int wave = tid >> 6;
+    const int j = lane & 15;   // MFMA row/column within the 16x16 tile
+    const int kk = lane >> 4;  // MFMA K index / row group
+
+    const uint64_t dim = 1ull << n;
+    const uint32_t qmask = blk->qmask;  // the block's 4 qubits
+    const uint32_t cmask = blk->cmask;  // super-tile column bits (qmask + lowest free)
+    const int m = __builtin_popcount(cmask);
+    const int ntiles = 1 << (m - 4);
+
+    // positions (inside the super-tile column index) of the block's qubits and
+    // of the free "batch" bits
+    uint32_t pos_q = 0, pos_f = 0;
+    {
+        int p = 0;
+        for (uint32_t b = cmask; b; b &= b - 1, p++) {
+            const uint32_t bit = b & (~b + 1);
+            if (bit & qmask)
+                pos_q |= 1u << p;
+            else
+                pos_f |= 1u << p;
+        }
+    }
+
+    double ur[4], ui[4], nui[4];
+#pragma unroll
+    for (int c = 0; c < 4; c++) {
+        const int idx = j * 16 + 4 * c + kk;
+        ur[c] = blk->ur[idx];
+        ui[c] = blk->ui[idx];
+        nui[c] = -ui[c];
+    }
+    // super-tile column index of tile element b (its Q part)
+    uint32_t lcq_a[4];
+#pragma unroll
+    for (int c = 0; c < 4; c++) lcq_a[c] = (uint32_t)deposit(4 * c + kk, pos_q);
+    const uint32_t lcq_d = (uint32_t)deposit(j, pos_q);
+
+    const uint64_t row_free = (dim - 1) & ~(uint64_t)qmask;
+    const uint64_t col_free = (dim - 1) & ~(uint64_t)cmask;
+    const uint64_t n_cg = dim >> m;
+    const uint64_t n_st = (dim >> 4) * n_cg;
+    const int per_plane = 16 << m;
+
+    // this thread's fixed super-tile column (the same for every row it moves)
+    const int lc = (2 * tid) & ((1 << m) - 1);
+    const uint64_t coloff = deposit((uint64_t)lc, cmask);
+
+    for (uint64_t st = blockIdx.x; st < n_st; st += gridDim.x) {
+        const uint64_t rg = st / n_cg;
+        const uint64_t cg = st - rg * n_cg;
+        const uint64_t r0 = deposit(rg, row_free);
+        const uint64_t c0 = deposit(cg, col_free) + coloff;
+
+        for (int e = 2 * tid; e < per_plane; e += 512) {
+            const int a = e >> m;
+            const uint64_t g = (r0 + deposit((uint64_t)a, qmask)) * dim + c0;
+            const double2 vr = *reinterpret_cast<const double2*>(re + g);
+            const double2 vi = *reinterpret_cast<const double2*>(im + g);
+            s_re[a * kPitch + lc] = vr.x;
+            s_re[a * kPitch + lc + 1] = vr.y;
+            s_im[a * kPitch + lc] = vi.x;
+            s_im[a * kPitch + lc + 1] = vi.y;
+        }
+        __syncthreads();
+
+        for (int tb = wave; tb < ntiles; tb += 4) {
+            const uint32_t base = (uint32_t)deposit((uint64_t)tb, pos_f);
+            double tr[4], ti[4];
+#pragma unroll
+            for (int c = 0; c < 4; c++) {
+                tr[c] = s_re[j * kPitch + base + lcq_a[c]];
+                ti[c] = s_im[j * kPitch + base + lcq_a[c]];
+            }
+            d4 xr = {0, 0, 0, 0}, xi = {0, 0, 0, 0};
+#pragma unroll
+            for (int c = 0; c < 4; c++) {
+                xr = mfma(tr[c], ur[c], xr);
+                xr = mfma(ti[c], nui[c], xr);
+                xi = mfma(tr[c], ui[c], xi);
+                xi = mfma(ti[c], ur[c], xi);
+            }
+            d4 yr = {0, 0, 0, 0}, yi = {0, 0, 0, 0};
+#pragma unroll
+            for (int c = 0; c < 4; c++) {
+                yr = mfma(ur[c], xr[c], yr);
+                yr = mfma(ui[c], xi[c], yr);
+                yi = mfma(ur[c], xi[c], yi);
+                yi = mfma(nui[c], xr[c], yi);
+            }
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const int a = 4 * r + kk;
+                s_re[a * kPitch + base + lcq_d] = yr[r];
+                s_im[a * kPitch + base + lcq_d] = yi[r];
+            }
+        }
+        __syncthreads();
+
+        for (int e = 2 * tid; e < per_plane; e += 512) {
+            const int a = e >> m;
+            const uint64_t g = (r0 + deposit((uint64_t)a, qmask)) * dim + c0;
+            *reinterpret_cast<double2*>(re + g) =
+                make_double2(s_re[a * kPitch + lc], s_re[a * kPitch + lc + 1]);
+            *reinterpret_cast<double2*>(im + g) =
+                make_double2(s_im[a * kPitch + lc], s_im[a * kPitch + lc + 1]);
+        }
+        __syncthreads();
+    }
+}
+
+hipError_t launch_fused4(double* re, double* im, const FusedBlock* d_blk, int n, int max_blocks,
+                         hipStream_t stream)
+{
+    if (n < 4) return hipErrorInvalidValue;
+    const int m = n < 8 ? n : 8;
+    const uint64_t n_st = (1ull << (n - 4)) * (1ull << (n - m));
+    const unsigned blocks = (unsigned)(n_st < (uint64_t)max_blocks ? n_st : (uint64_t)max_blocks);
+    k_fused4<<<blocks, 256, 0, stream>>>(re, im, d_blk, n);
+    return hipGetLastError();
+}
+
+}  // namespace dmx

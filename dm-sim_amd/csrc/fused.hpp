@@ -1,0 +1,23 @@
+// fused.hpp -- device record and launcher of the dense 4-qubit block sweep.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace dmx {
+
+// A dense 4-qubit block U (16 x 16, row-major, local index bit t <-> the t-th
+// lowest qubit of qmask) and the super-tile column bits used to stage it.
+struct FusedBlock {
+    uint32_t qmask;  // exactly four qubits
+    uint32_t cmask;  // qmask plus the lowest free column bits, min(8, n) bits
+    uint32_t pad[2];
+    double ur[256];
+    double ui[256];
+};
+
+// rho <- U rho U^dagger for one block (n >= 4).
+hipError_t launch_fused4(double* re, double* im, const FusedBlock* d_blk, int n, int max_blocks,
+                         hipStream_t stream);
+
+}  // namespace dmx

@@ -1,0 +1,187 @@
+"""GPU checks of the fused method (dense 4-qubit blocks, FP64 MFMA, both sides
+of rho in one sweep).  Fusing reorders floating-point work, so the bar is
+max|d rho| <= 1e-10 against the oracle (the reference CPU path), with trace and
+Hermiticity to 1e-12; circuits the planner cannot fuse (SRN, n < 4) fall back
+to the plain sweeps and stay bit-exact."""
+import math
+import os
+
+import numpy as np
+import pytest
+
+import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-10
+FUSED = 2
+
+
+def dmx():
+    import dmsim_amd
+    return dmsim_amd
+
+
+def run(n, tables, method=FUSED, max_k=4):
+    s = dmx().Sim(n)
+    s.set_method(method)
+    s.set_fusion_max_k(max_k)
+    for t in tables:
+        s.clear_circuit()
+        s.append_table(t)
+        s.upload()
+        s.run()
+    return s
+
+
+def check_close(s, re_ref, im_ref, n):
+    re, im = s.get_dm()
+    err = max(np.abs(re - re_ref).max(), np.abs(im - im_ref).max())
+    assert err <= TOL, err
+    rho = (re + 1j * im).reshape(1 << n, 1 << n)
+    assert abs(np.trace(rho) - np.trace((re_ref + 1j * im_ref).reshape(1 << n, 1 << n))) < 1e-12
+    assert np.max(np.abs(rho - rho.conj().T)) < 1e-12
+    return err
+
+
+UNITARY_SET = [g for g in W.OP_NAMES if g not in ("SRN", "R", "C4X")]
+
+
+@pytest.mark.parametrize("n", [4, 5, 6, 8, 9])
+@pytest.mark.parametrize("max_k", [2, 3, 4])
+def test_fused_random_vs_oracle(oracle, n, max_k):
+    gate_set = [g for g in UNITARY_SET if W.ARITY[g] <= min(n, 4)]
+    t = W.random_mixed(n, 300, 3000 + 10 * n + max_k, gate_set=gate_set)
+    s = run(n, [t], max_k=max_k)
+    st = s.last_run_stats()
+    assert 0 < st["launches"] <= len(t)
+    re, im = oracle.sim(n, t.arrays())
+    check_close(s, re, im, n)
+
+
+def test_fused_non_unitary_r_gate(oracle):
+    """R(theta) is linear but not unitary; the fused matrix keeps it."""
+    n = 5
+    gate_set = ["R", "H", "CX", "U3", "RZZ"]
+    t = W.random_mixed(n, 60, 77, gate_set=gate_set)
+    s = run(n, [t])
+    re, im = oracle.sim(n, t.arrays())
+    scale = max(1.0, np.abs(re).max(), np.abs(im).max())
+    got_re, got_im = s.get_dm()
+    assert max(np.abs(got_re - re).max(), np.abs(got_im - im).max()) <= TOL * scale
+
+
+def test_fused_matrix_gates_dense_brute_force():
+    """3- and 4-qubit matrix gates (fused-only) against numpy's conj(M rho M^+)."""
+    n = 5
+    rng = np.random.default_rng(5)
+    s = dmx().Sim(n)
+    s.set_method(FUSED)
+    M = np.eye(1 << n, dtype=complex)
+    for _ in range(6):
+        k = int(rng.integers(1, 5))
+        qs = [int(x) for x in rng.choice(n, size=k, replace=False)]
+        u = W.haar_unitary(rng, 1 << k)
+        s.append_matrix(qs, u)
+        full = np.zeros((1 << n, 1 << n), dtype=complex)
+        for x in range(1 << n):
+            sub_in = sum(((x >> q) & 1) << (k - 1 - jj) for jj, q in enumerate(qs))
+            rest = x & ~sum(1 << q for q in qs)
+            for so in range(1 << k):
+                y = rest | sum(((so >> (k - 1 - jj)) & 1) << q for jj, q in enumerate(qs))
+                full[y, x] = u[so, sub_in]
+        M = full @ M
+    s.upload()
+    s.run()
+    rho0 = np.zeros((1 << n, 1 << n))
+    rho0[0, 0] = 1
+    want = np.conj(M @ rho0 @ M.conj().T)
+    re, im = s.get_dm()
+    got = (re + 1j * im).reshape(1 << n, 1 << n)
+    assert np.max(np.abs(got - want)) <= TOL
+
+
+def test_fused_c4x_runs_as_plain_segment(oracle):
+    """C4X spans 5 qubits, above the block width: it runs as a plain segment
+    between fused blocks, keeping the reference's (non-textbook) semantics."""
+    n = 6
+    gate_set = ["H", "CX", "U3", "CCX", "C4X", "RZZ", "C3X"]
+    t = W.random_mixed(n, 40, 4, gate_set=gate_set)
+    assert W.OP["C4X"] in t.ops
+    s = run(n, [t])
+    re, im = oracle.sim(n, t.arrays())
+    check_close(s, re, im, n)
+
+
+def test_fused_srn_falls_back_bitwise(oracle):
+    n = 5
+    t = W.random_mixed(n, 80, 12, gate_set=["SRN", "H", "CX", "U3"])
+    s = run(n, [t])
+    re, im = s.get_dm()
+    ore, oim = oracle.sim(n, t.arrays())
+    assert np.array_equal(re, ore) and np.array_equal(im, oim)
+
+
+def test_mixed_methods_continuation(oracle):
+    """plain run, fused run, plain run on one state."""
+    n = 7
+    segs = [W.random_mixed(n, 60, 40 + i, gate_set=UNITARY_SET[:30]) for i in range(3)]
+    s = dmx().Sim(n)
+    state = None
+    for i, t in enumerate(segs):
+        s.set_method(FUSED if i == 1 else 0)
+        s.clear_circuit()
+        s.append_table(t)
+        s.upload()
+        s.run()
+        state = oracle.sim(n, t.arrays(), state)
+    check_close(s, state[0], state[1], n)
+
+
+def test_fused_brickwall_n12_vs_oracle(oracle):
+    oracle.set_threads(min(os.cpu_count() or 1, 16))
+    t = W.brickwall_su4(n=12, layers=6, tail=60, seed=12)
+    s = run(12, [t])
+    re, im = oracle.sim(12, t.arrays())
+    check_close(s, re, im, 12)
+
+
+def test_fused_n14_vs_state_vector_and_plain(oracle):
+    n = 14
+    t = W.brickwall_su4(n=n, layers=20, tail=2000, seed=14)
+    s = run(n, [t])
+    st = s.last_run_stats()
+    psi = oracle.sv(n, t.arrays())
+    diag = s.get_diag()
+    assert np.max(np.abs(diag - np.abs(psi) ** 2)) < TOL
+    assert abs(diag.sum() - 1) < 1e-12
+    re, im = s.get_dm()
+    dim = 1 << n
+    rng = np.random.default_rng(1)
+    i = rng.integers(0, dim, 8192)
+    j = rng.integers(0, dim, 8192)
+    got = re[i * dim + j] + 1j * im[i * dim + j]
+    assert np.max(np.abs(got - np.conj(psi[i]) * psi[j])) < TOL
+    assert st["launches"] < len(t) // 2  # fusion actually merged gates
+    del re, im
+    p = run(n, [t], method=1)
+    pre, pim = p.get_dm()
+    fre, fim = s.get_dm()
+    assert max(np.abs(pre - fre).max(), np.abs(pim - fim).max()) < TOL
+
+
+def test_fused_shots_match_plain_on_known_answer():
+    first, second = W.adder_n10()
+    s = run(10, [first, second])
+    shots, total = s.measure(10, 7)
+    assert list(shots) == [514] * 10 and abs(total - 1) < 1e-12
+
+
+def test_fused_deuteron():
+    for tval in np.linspace(-math.pi, math.pi, 9):
+        g = W.GateTable(4).add("X", [0]).add("RY", [1], [tval]).add("CX", [1, 0])
+        g.add("H", [0]).add("H", [1])
+        s = run(4, [g])
+        d = s.get_diag()
+        zz = sum(d[i] * (1 if bin(i & 3).count("1") % 2 == 0 else -1) for i in range(16))
+        assert abs(zz - math.sin(tval)) < 1e-12
