@@ -8,8 +8,14 @@ each step continues from the previous state (the reference's
 clear_circuit/upload/run continuation semantics).
 
 Prints one JSON line.  `value` is API gate-updates/s of the headline method
-(whole-job, device-resident data); `roofline` describes the dominant kernel;
-`cpu_baseline` times the oracle's CPU restatement on a bounded sample.
+(whole-job, device-resident data).  `roofline` describes the kernel class that
+takes the most device time in the headline method: its algorithmic bytes per
+launch over its mean launch time, both from HIP events recorded around every
+launch on the run's stream during the timed steps.  `cpu_baseline` times the
+oracle's CPU restatement of the reference path on a bounded sample.
+
+  python bench.py                      # default: fused and plain, n = 14
+  python bench.py --calibrate-copy     # one 2 x 2 GiB device copy (counter scaling)
 """
 import argparse
 import json
@@ -23,6 +29,11 @@ sys.path.insert(0, os.path.join(ROOT, "dm-sim_amd"))
 
 METRIC = "ρ gate-updates/s at n=14 FP64: fused+batched vs plain gate-by-gate; HBM GB/s"
 HBM_PEAK_GBS = 8000.0
+# HBM bytes per launch over algorithmic bytes per launch, from rocprofv3 FETCH_SIZE and
+# WRITE_SIZE runs of this workload scaled by a copy of known size
+# (profiles/k_fused4_n14_hbm_counters.md).
+TRAFFIC_RATIO = {"k_fused4": 8.591e9 / 8.590e9}
+METHODS = {"plain": 0, "plain_launch": 1, "fused": 2}
 
 
 def log(msg):
@@ -32,28 +43,32 @@ def log(msg):
 def timed_runs(sim, steps, warmup):
     for _ in range(warmup):
         sim.run()
+    sim.set_profiling(True)
     dev_ms = []
+    prof = {}
     t0 = time.perf_counter()
     for _ in range(steps):
         sim.run()
         dev_ms.append(sim.last_run_stats()["ms"])
+        for k, v in sim.profile().items():
+            acc = prof.setdefault(k, {"calls": 0, "ms": 0.0, "bytes": 0.0})
+            for f in acc:
+                acc[f] += v[f]
     wall = time.perf_counter() - t0
-    return wall, dev_ms, sim.last_run_stats()
+    sim.set_profiling(False)
+    return wall, dev_ms, sim.last_run_stats(), prof
 
 
-def cpu_baseline(table, n, seconds_hint=15.0):
-    """The oracle (C restatement of the reference CPU path) on the first gates
-    of the same workload, from |0><0|, on this host's cores."""
-    import numpy as np
-    import workloads as W
+def cpu_baseline(table, n, gates=120):
+    """The oracle (C restatement of the reference CPU path) on a slice of the
+    same workload's U3/CX tail, from |0><0|, on this host's cores."""
     from oracle.oracle import Oracle
 
     threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or min(os.cpu_count() or 1, 16)
     o = Oracle()
     o.set_threads(threads)
-    # skip the SU(4) prefix: the sample is 12 gates of the U3/CX tail
     start = 130
-    sample = table.slice(start, start + 12)
+    sample = table.slice(start, start + gates)
     t0 = time.perf_counter()
     o.sim(n, sample.arrays())
     dt = time.perf_counter() - t0
@@ -62,9 +77,22 @@ def cpu_baseline(table, n, seconds_hint=15.0):
         "unit": "gate-updates/s",
         "cores": threads,
         "kind": "port",
-        "sample": f"oracle sim() of gates {start}..{start + len(sample) - 1} (U3/CX) of the "
-                  f"n={n} workload from |0><0|, {dt:.1f} s",
+        "sample": f"oracle sim() of gates {start}..{start + len(sample) - 1} (U3/CX tail) of the "
+                  f"n={n} workload from |0><0| with {threads} OpenMP threads: {dt:.1f} s",
     }
+
+
+def calibrate_copy():
+    """A plain device-to-device copy of 2 GiB per plane pair (4 GiB read +
+    4 GiB written in total) for scaling the FETCH_SIZE/WRITE_SIZE counters."""
+    import torch
+    a = torch.empty(2 ** 28, dtype=torch.float64, device="cuda")
+    b = torch.empty_like(a)
+    a.fill_(1.0)
+    torch.cuda.synchronize()
+    b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes_read": a.numel() * 8, "copy_bytes_written": b.numel() * 8}))
 
 
 def main():
@@ -75,11 +103,15 @@ def main():
     ap.add_argument("--n", type=int, default=14)
     ap.add_argument("--methods", default="fused,plain_launch")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--calibrate-copy", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
               "(run with --gpus 1)", file=sys.stderr)
         sys.exit(2)
+    if args.calibrate_copy:
+        calibrate_copy()
+        return
 
     import dmsim_amd
     import workloads as W
@@ -89,14 +121,14 @@ def main():
     gates = len(table)
     num = 4 ** n
     results = {}
+    profiles = {}
     for mname in args.methods.split(","):
-        method = {"plain": 0, "plain_launch": 1, "fused": 2}[mname]
         sim = dmsim_amd.Sim(n)
-        sim.set_method(method)
+        sim.set_method(METHODS[mname])
         sim.append_table(table)
         sim.upload()
         log(f"[bench] {mname}: n={n} gates={gates} warmup={args.warmup} steps={args.steps}")
-        wall, dev_ms, stats = timed_runs(sim, args.steps, args.warmup)
+        wall, dev_ms, stats, prof = timed_runs(sim, args.steps, args.warmup)
         ms_step = wall * 1e3 / args.steps
         dev_avg = sum(dev_ms) / len(dev_ms)
         results[mname] = {
@@ -104,17 +136,25 @@ def main():
             "ms_per_step": ms_step,
             "device_ms_per_step": dev_avg,
             "launches_per_step": stats["launches"],
-            "sweeps_per_step": stats["sweeps"],
+            "gates_per_launch": gates / max(stats["launches"], 1),
+            # 64 * 4^n bytes per gate-update (two full sweeps in the plain method)
+            "effective_GBs": 64.0 * num * gates / (dev_avg * 1e-3) / 1e9,
         }
+        profiles[mname] = prof
         log(f"[bench] {mname}: {ms_step:.1f} ms/step, device {dev_avg:.1f} ms, "
             f"{results[mname]['gate_updates_per_s']:.1f} gate-updates/s")
+        for k, v in sorted(prof.items(), key=lambda kv: -kv[1]["ms"]):
+            log(f"[bench]   {k:24s} calls={v['calls']:6d} mean={v['ms'] / v['calls'] * 1e3:9.1f} us "
+                f"{v['bytes'] / (v['ms'] * 1e-3) / 1e9:8.1f} GB/s")
         sim.close()
 
     best = max(results, key=lambda k: results[k]["gate_updates_per_s"])
     r = results[best]
-    # plain method: two sweeps per gate-update, 32*4^n bytes each (64*4^n B/gate)
-    alg_bytes = 64.0 * num * gates
-    achieved = alg_bytes / (r["device_ms_per_step"] * 1e-3) / 1e9
+    prof = profiles[best]
+    kname, kv = max(prof.items(), key=lambda kv: kv[1]["ms"])
+    per_launch_bytes = kv["bytes"] / kv["calls"]
+    mean_s = kv["ms"] / kv["calls"] * 1e-3
+    achieved = per_launch_bytes / mean_s / 1e9
     line = {
         "metric": METRIC,
         "value": r["gate_updates_per_s"],
@@ -134,14 +174,20 @@ def main():
             "n_qubits": n, "gates": gates, "method": best,
         },
         "methods": results,
+        "speedup_vs_plain": (r["gate_updates_per_s"] / results["plain_launch"]["gate_updates_per_s"]
+                             if "plain_launch" in results else None),
         "roofline": {
             "bound": "hbm",
             "achieved": achieved,
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
-            "kernel": "plain sweep family (64*4^n B per gate-update over the run's device time)",
+            "traffic": (TRAFFIC_RATIO[kname] * per_launch_bytes if kname in TRAFFIC_RATIO
+                        else None),
+            "kernel": kname,
+            "bytes_per_launch": per_launch_bytes,
+            "mean_launch_us": mean_s * 1e6,
+            "launches": kv["calls"],
         },
     }
     if not args.no_cpu_baseline:

@@ -220,6 +220,33 @@ int dmx_last_run_stats(const dmx_sim* sim, double* ms, uint32_t* launches, uint3
     return DMX_OK;
 }
 
+int dmx_set_profiling(dmx_sim* sim, int on)
+{
+    if (!sim) return null_handle();
+    sim->eng.profiler().enable(on != 0);
+    return DMX_OK;
+}
+
+int dmx_profile_get(dmx_sim* sim, uint32_t cls, char* name, size_t cap, uint32_t* calls,
+                    double* ms, double* bytes, uint32_t* count)
+{
+    if (!sim) return null_handle();
+    if (count) *count = dmx::KC_COUNT;
+    if (cls >= (uint32_t)dmx::KC_COUNT) {
+        dmx::set_error("kernel class index out of range");
+        return DMX_EINVAL;
+    }
+    const dmx::KernelStat& st = sim->eng.profiler().stat((int)cls);
+    if (name && cap > 0) {
+        std::strncpy(name, dmx::kKernelClassNames[cls], cap - 1);
+        name[cap - 1] = '\0';
+    }
+    if (calls) *calls = st.calls;
+    if (ms) *ms = st.ms;
+    if (bytes) *bytes = st.bytes;
+    return DMX_OK;
+}
+
 const char* dmx_last_error(void) { return dmx::get_error(); }
 
 }  // extern "C"

@@ -170,6 +170,7 @@ int Engine::run_plain(bool per_launch)
     const int n_ops = (int)prims_.size();
     last_launches_ = 0;
     last_sweeps_ = 2u * (uint32_t)n_ops;
+    prof_.start(stream_);
     HIPTRY(hipEventRecord(ev0_, stream_));
     if (n_ops > 0) {
         // reference orientation: fast index first; transposed: slow first
@@ -179,16 +180,21 @@ int Engine::run_plain(bool per_launch)
             int t = 0;
             while (t < n_ops) {
                 if (per_launch || prims_[t].kind == PK_C2) {
+                    prof_.begin(prim_class(prims_[t]), prim_bytes(prims_[t], n_));
                     HIPTRY(launch_plain_one(d_re_, d_im_, d_ops_ + t, prims_[t], n_, slow,
                                             max_blocks_, stream_));
+                    prof_.end();
                     last_launches_++;
                     t++;
                     continue;
                 }
                 int end = t;
-                while (end < n_ops && prims_[end].kind != PK_C2) end++;
+                double bytes = 0;
+                while (end < n_ops && prims_[end].kind != PK_C2) bytes += prim_bytes(prims_[end++], n_);
+                prof_.begin(KC_PERSISTENT, bytes);
                 HIPTRY(launch_plain_persistent(d_re_, d_im_, d_ops_, t, end, n_, slow,
                                                persistent_blocks_, stream_));
+                prof_.end();
                 last_launches_++;
                 t = end;
             }
@@ -200,6 +206,7 @@ int Engine::run_plain(bool per_launch)
     float ms = 0.f;
     HIPTRY(hipEventElapsedTime(&ms, ev0_, ev1_));
     last_ms_ = ms;
+    HIPTRY(prof_.collect());
     return DMX_OK;
 }
 
@@ -207,15 +214,16 @@ int Engine::run_fused()
 {
     last_launches_ = 0;
     last_sweeps_ = 0;
+    prof_.start(stream_);
     HIPTRY(hipEventRecord(ev0_, stream_));
-    int rc = plan_->execute(d_re_, d_im_, n_, transposed_, d_ops_, stream_, persistent_blocks_,
-                            &last_launches_, &last_sweeps_);
+    int rc = plan_->execute(d_re_, d_im_, n_, stream_, prof_, &last_launches_, &last_sweeps_);
     if (rc) return rc;
     HIPTRY(hipEventRecord(ev1_, stream_));
     HIPTRY(hipEventSynchronize(ev1_));
     float ms = 0.f;
     HIPTRY(hipEventElapsedTime(&ms, ev0_, ev1_));
     last_ms_ = ms;
+    HIPTRY(prof_.collect());
     return DMX_OK;
 }
 

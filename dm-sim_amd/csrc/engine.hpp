@@ -8,6 +8,7 @@
 
 #include "lower.hpp"
 #include "prim.hpp"
+#include "prof.hpp"
 
 namespace dmx {
 
@@ -40,6 +41,7 @@ public:
     double last_ms() const { return last_ms_; }
     uint32_t last_launches() const { return last_launches_; }
     uint32_t last_sweeps() const { return last_sweeps_; }
+    LaunchProfiler& profiler() { return prof_; }
 
 private:
     int run_plain(bool per_launch);
@@ -68,6 +70,7 @@ private:
     std::vector<Gate> circuit_;
     std::vector<PrimOp> prims_;
     FusedPlan* plan_ = nullptr;
+    LaunchProfiler prof_;
 
     double last_ms_ = 0.0;
     uint32_t last_launches_ = 0;

@@ -75,20 +75,25 @@ struct Plan : FusedPlan {
         return rc ? rc : put(&d_prims, prims, stream);
     }
 
-    int execute(double* re, double* im, int nq, bool&, PrimOp*, hipStream_t stream, int,
+    int execute(double* re, double* im, int nq, hipStream_t stream, LaunchProfiler& prof,
                 uint32_t* launches, uint32_t* sweeps) override
     {
+        const double sweep_bytes = 32.0 * (double)(1ull << (2 * nq));
         uint32_t la = 0;
         for (const Step& st : steps) {
             hipError_t e = hipSuccess;
             if (st.fused) {
+                prof.begin(KC_FUSED4, sweep_bytes);
                 e = launch_fused4(re, im, d_blocks + st.index, nq, max_blocks, stream);
+                prof.end();
                 la++;
             } else {
                 for (int side = 0; side < 2 && e == hipSuccess; side++)
                     for (int t = st.index; t < st.index + st.count && e == hipSuccess; t++) {
+                        prof.begin(prim_class(prims[t]), prim_bytes(prims[t], nq));
                         e = launch_plain_one(re, im, d_prims + t, prims[t], nq, side == 1, 8192,
                                              stream);
+                        prof.end();
                         la++;
                     }
             }

@@ -8,17 +8,16 @@
 
 #include "lower.hpp"
 #include "prim.hpp"
+#include "prof.hpp"
 
 namespace dmx {
 
 struct FusedPlan {
     virtual ~FusedPlan() = default;
     virtual int upload(hipStream_t stream) = 0;
-    // Applies the whole circuit once; `transposed` is the engine's orientation
-    // flag (plain primitive segments flip it).
-    virtual int execute(double* re, double* im, int n, bool& transposed, PrimOp* d_scratch_ops,
-                        hipStream_t stream, int persistent_blocks, uint32_t* launches,
-                        uint32_t* sweeps) = 0;
+    // Applies the whole circuit once on `stream`, timing launches into prof.
+    virtual int execute(double* re, double* im, int n, hipStream_t stream, LaunchProfiler& prof,
+                        uint32_t* launches, uint32_t* sweeps) = 0;
 };
 
 int build_fused_plan(const std::vector<Gate>& circuit, int n, int max_k, FusedPlan** out);

@@ -8,8 +8,10 @@
 // free column bits, so every global access is a contiguous run of at least 16
 // doubles (the four lowest column bits are always inside the super-tile).
 // Each of the 16 tiles of a super-tile is transformed by one wave:
-//   X  = T U^T      (16 MFMAs, complex as four real products)
-//   T' = conj(U) X  (16 MFMAs)
+//   X  = T U^T      (12 MFMAs: complex product from three real products)
+//   T' = conj(U) X  (12 MFMAs)
+// Two workgroups per CU (64 KiB of LDS each) overlap one another's load,
+// transform and store phases.
 // MFMA lane layout on gfx950 (measured): A[i][k] in lane i+16k, B[k][j] in
 // lane j+16k, D[4r+g][j] in lane j+16g register r -- so register c of X is
 // exactly the B operand of the second product's K-chunk c, and the block
@@ -48,6 +50,8 @@ __device__ __forceinline__ uint64_t deposit(uint64_t v, uint64_t mask)
 
 }  // namespace
 
+// M = number of super-tile column bits (8 for n >= 8, n below that).
+template <int M>
 __global__ void __launch_bounds__(256, 2) k_fused4(double* __restrict__ re, double* __restrict__ im,
                                                     const FusedBlock* __restrict__ blk, int n)
 {
@@ -62,9 +66,11 @@ __global__ void __launch_bounds__(256, 2) k_fused4(double* __restrict__ re, doub
 
     const uint64_t dim = 1ull << n;
     const uint32_t qmask = blk->qmask;  // the block's 4 qubits
-    const uint32_t cmask = blk->cmask;  // super-tile column bits (qmask + lowest free)
-    const int m = __builtin_popcount(cmask);
-    const int ntiles = 1 << (m - 4);
+    // super-tile column bits: the block's qubits plus the lowest free bits, M in all
+    uint32_t cmask = qmask;
+    for (int q = 0; q < n && __builtin_popcount(cmask) < M; q++) cmask |= 1u << q;
+    constexpr int m = M;
+    constexpr int ntiles = 1 << (M - 4);
 
     // positions (inside the super-tile column index) of the block's qubits and
     // of the free "batch" bits
@@ -80,13 +86,19 @@ __global__ void __launch_bounds__(256, 2) k_fused4(double* __restrict__ re, doub
         }
     }
 
-    double ur[4], ui[4], nui[4];
+    // Complex products with three real multiplications each (P1 = Ar Br,
+    // P2 = Ai Bi, P3 = (Ar + Ai)(Br + Bi); re = P1 - P2, im = P3 - P1 - P2).
+    // First product B = U^T, second product A = conj(U): both read U[j][4c+k]
+    // in lane j+16k.
+    double ur[4], ui[4], upi[4], nui[4], umi[4];
 #pragma unroll
     for (int c = 0; c < 4; c++) {
         const int idx = j * 16 + 4 * c + kk;
         ur[c] = blk->ur[idx];
         ui[c] = blk->ui[idx];
+        upi[c] = ur[c] + ui[c];
         nui[c] = -ui[c];
+        umi[c] = ur[c] - ui[c];
     }
     // super-tile column index of tile element b (its Q part)
     uint32_t lcq_a[4];
@@ -96,56 +108,76 @@ __global__ void __launch_bounds__(256, 2) k_fused4(double* __restrict__ re, doub
 
     const uint64_t row_free = (dim - 1) & ~(uint64_t)qmask;
     const uint64_t col_free = (dim - 1) & ~(uint64_t)cmask;
-    const uint64_t n_cg = dim >> m;
-    const uint64_t n_st = (dim >> 4) * n_cg;
-    const int per_plane = 16 << m;
+    const int cg_bits = n - m;
+    const uint64_t n_st = 1ull << (n - 4 + cg_bits);
+    constexpr int per_plane = 16 << M;
+    constexpr int iters = per_plane >= 512 ? per_plane / 512 : 1;
+    const bool active = 2 * tid < per_plane;
 
     // this thread's fixed super-tile column (the same for every row it moves)
     const int lc = (2 * tid) & ((1 << m) - 1);
     const uint64_t coloff = deposit((uint64_t)lc, cmask);
+    uint64_t rowoff[iters];
+#pragma unroll
+    for (int it = 0; it < iters; it++)
+        rowoff[it] = deposit((uint64_t)((2 * tid + 512 * it) >> m), qmask) * dim;
+
+    auto base_of = [&](uint64_t st) {
+        const uint64_t rg = st >> cg_bits;
+        const uint64_t cg = st & ((1ull << cg_bits) - 1);
+        return deposit(rg, row_free) * dim + deposit(cg, col_free) + coloff;
+    };
 
     for (uint64_t st = blockIdx.x; st < n_st; st += gridDim.x) {
-        const uint64_t rg = st / n_cg;
-        const uint64_t cg = st - rg * n_cg;
-        const uint64_t r0 = deposit(rg, row_free);
-        const uint64_t c0 = deposit(cg, col_free) + coloff;
-
-        for (int e = 2 * tid; e < per_plane; e += 512) {
-            const int a = e >> m;
-            const uint64_t g = (r0 + deposit((uint64_t)a, qmask)) * dim + c0;
-            const double2 vr = *reinterpret_cast<const double2*>(re + g);
-            const double2 vi = *reinterpret_cast<const double2*>(im + g);
-            s_re[a * kPitch + lc] = vr.x;
-            s_re[a * kPitch + lc + 1] = vr.y;
-            s_im[a * kPitch + lc] = vi.x;
-            s_im[a * kPitch + lc + 1] = vi.y;
+        const uint64_t g0 = base_of(st);
+        if (active) {
+            double2 vr[iters], vi[iters];
+#pragma unroll
+            for (int it = 0; it < iters; it++) {
+                vr[it] = *reinterpret_cast<const double2*>(re + g0 + rowoff[it]);
+                vi[it] = *reinterpret_cast<const double2*>(im + g0 + rowoff[it]);
+            }
+#pragma unroll
+            for (int it = 0; it < iters; it++) {
+                const int a = (2 * tid + 512 * it) >> m;
+                s_re[a * kPitch + lc] = vr[it].x;
+                s_re[a * kPitch + lc + 1] = vr[it].y;
+                s_im[a * kPitch + lc] = vi[it].x;
+                s_im[a * kPitch + lc + 1] = vi[it].y;
+            }
         }
         __syncthreads();
 
         for (int tb = wave; tb < ntiles; tb += 4) {
             const uint32_t base = (uint32_t)deposit((uint64_t)tb, pos_f);
-            double tr[4], ti[4];
+            double tr[4], ti[4], ts[4];
 #pragma unroll
             for (int c = 0; c < 4; c++) {
                 tr[c] = s_re[j * kPitch + base + lcq_a[c]];
                 ti[c] = s_im[j * kPitch + base + lcq_a[c]];
+                ts[c] = tr[c] + ti[c];
             }
-            d4 xr = {0, 0, 0, 0}, xi = {0, 0, 0, 0};
+            // X = T U^T
+            d4 p1 = {0, 0, 0, 0}, p2 = {0, 0, 0, 0}, p3 = {0, 0, 0, 0};
 #pragma unroll
             for (int c = 0; c < 4; c++) {
-                xr = mfma(tr[c], ur[c], xr);
-                xr = mfma(ti[c], nui[c], xr);
-                xi = mfma(tr[c], ui[c], xi);
-                xi = mfma(ti[c], ur[c], xi);
+                p1 = mfma(tr[c], ur[c], p1);
+                p2 = mfma(ti[c], ui[c], p2);
+                p3 = mfma(ts[c], upi[c], p3);
             }
-            d4 yr = {0, 0, 0, 0}, yi = {0, 0, 0, 0};
+            const d4 xr = p1 - p2;
+            const d4 xi = p3 - p1 - p2;
+            const d4 xs = xr + xi;
+            // T' = conj(U) X, conj(U) = ur + i*(-ui)
+            d4 q1 = {0, 0, 0, 0}, q2 = {0, 0, 0, 0}, q3 = {0, 0, 0, 0};
 #pragma unroll
             for (int c = 0; c < 4; c++) {
-                yr = mfma(ur[c], xr[c], yr);
-                yr = mfma(ui[c], xi[c], yr);
-                yi = mfma(ur[c], xi[c], yi);
-                yi = mfma(nui[c], xr[c], yi);
+                q1 = mfma(ur[c], xr[c], q1);
+                q2 = mfma(nui[c], xi[c], q2);
+                q3 = mfma(umi[c], xs[c], q3);
             }
+            const d4 yr = q1 - q2;
+            const d4 yi = q3 - q1 - q2;
 #pragma unroll
             for (int r = 0; r < 4; r++) {
                 const int a = 4 * r + kk;
@@ -155,13 +187,15 @@ __global__ void __launch_bounds__(256, 2) k_fused4(double* __restrict__ re, doub
         }
         __syncthreads();
 
-        for (int e = 2 * tid; e < per_plane; e += 512) {
-            const int a = e >> m;
-            const uint64_t g = (r0 + deposit((uint64_t)a, qmask)) * dim + c0;
-            *reinterpret_cast<double2*>(re + g) =
-                make_double2(s_re[a * kPitch + lc], s_re[a * kPitch + lc + 1]);
-            *reinterpret_cast<double2*>(im + g) =
-                make_double2(s_im[a * kPitch + lc], s_im[a * kPitch + lc + 1]);
+        if (active) {
+#pragma unroll
+            for (int it = 0; it < iters; it++) {
+                const int a = (2 * tid + 512 * it) >> m;
+                *reinterpret_cast<double2*>(re + g0 + rowoff[it]) =
+                    make_double2(s_re[a * kPitch + lc], s_re[a * kPitch + lc + 1]);
+                *reinterpret_cast<double2*>(im + g0 + rowoff[it]) =
+                    make_double2(s_im[a * kPitch + lc], s_im[a * kPitch + lc + 1]);
+            }
         }
         __syncthreads();
     }
@@ -174,7 +208,13 @@ hipError_t launch_fused4(double* re, double* im, const FusedBlock* d_blk, int n,
     const int m = n < 8 ? n : 8;
     const uint64_t n_st = (1ull << (n - 4)) * (1ull << (n - m));
     const unsigned blocks = (unsigned)(n_st < (uint64_t)max_blocks ? n_st : (uint64_t)max_blocks);
-    k_fused4<<<blocks, 256, 0, stream>>>(re, im, d_blk, n);
+    switch (m) {
+    case 4: k_fused4<4><<<blocks, 256, 0, stream>>>(re, im, d_blk, n); break;
+    case 5: k_fused4<5><<<blocks, 256, 0, stream>>>(re, im, d_blk, n); break;
+    case 6: k_fused4<6><<<blocks, 256, 0, stream>>>(re, im, d_blk, n); break;
+    case 7: k_fused4<7><<<blocks, 256, 0, stream>>>(re, im, d_blk, n); break;
+    default: k_fused4<8><<<blocks, 256, 0, stream>>>(re, im, d_blk, n); break;
+    }
     return hipGetLastError();
 }
 

@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include "prim.hpp"
+#include "prof.hpp"
 
 namespace dmx {
 
@@ -18,6 +19,21 @@ hipError_t launch_plain_persistent(double* re, double* im, const PrimOp* d_ops, 
 // One primitive sweep; host_op is the host copy of *d_op (selects the kernel).
 hipError_t launch_plain_one(double* re, double* im, const PrimOp* d_op, const PrimOp& host_op,
                             int n, bool slow, int max_blocks, hipStream_t stream);
+
+// Kernel class and algorithmic bytes (entries a primitive reads and writes,
+// both planes) of one primitive sweep over 4^n entries.
+inline int prim_class(const PrimOp& p)
+{
+    if (p.kind == PK_C2) return KC_QUAD;
+    if (p.kind == PK_X && p.q1 >= 0) return KC_CX;
+    return pos1_only(p.kind) ? KC_PAIR_POS1 : KC_PAIR_DENSE;
+}
+inline double prim_bytes(const PrimOp& p, int n)
+{
+    const double full = 32.0 * (double)(1ull << (2 * n));
+    const int c = prim_class(p);
+    return (c == KC_CX || c == KC_PAIR_POS1) ? full / 2 : full;
+}
 
 // In-place rho <- conj(rho^T).
 hipError_t launch_conj_transpose(double* re, double* im, int n, hipStream_t stream);

@@ -23,7 +23,8 @@ EXPORTS = [
     "dmx_create", "dmx_destroy", "dmx_append", "dmx_append_matrix", "dmx_upload", "dmx_run",
     "dmx_clear_circuit", "dmx_reset", "dmx_reset_dm", "dmx_measure", "dmx_get_dm",
     "dmx_get_diag", "dmx_expect_z", "dmx_info", "dmx_dump", "dmx_set_method",
-    "dmx_set_fusion_max_k", "dmx_last_run_stats", "dmx_last_error",
+    "dmx_set_fusion_max_k", "dmx_last_run_stats", "dmx_set_profiling", "dmx_profile_get",
+    "dmx_last_error",
 ]
 
 _lib = None
@@ -61,6 +62,9 @@ def load():
     lib.dmx_set_method.argtypes = [vp, C.c_int]
     lib.dmx_set_fusion_max_k.argtypes = [vp, C.c_uint32]
     lib.dmx_last_run_stats.argtypes = [vp, P(C.c_double), P(C.c_uint32), P(C.c_uint32)]
+    lib.dmx_set_profiling.argtypes = [vp, C.c_int]
+    lib.dmx_profile_get.argtypes = [vp, C.c_uint32, C.c_char_p, C.c_size_t, P(C.c_uint32),
+                                    P(C.c_double), P(C.c_double), P(C.c_uint32)]
     lib.dmx_last_error.restype = C.c_char_p
     for name in EXPORTS:
         if name != "dmx_last_error":
@@ -190,6 +194,25 @@ class Sim:
 
     def set_fusion_max_k(self, k: int):
         _check(load().dmx_set_fusion_max_k(self._h, k))
+
+    def set_profiling(self, on: bool):
+        _check(load().dmx_set_profiling(self._h, 1 if on else 0))
+
+    def profile(self):
+        """{class name: {calls, ms, bytes}} of the last profiled run."""
+        lib = load()
+        cnt = C.c_uint32()
+        _check(lib.dmx_profile_get(self._h, 0, None, 0, None, None, None, C.byref(cnt)))
+        out = {}
+        for i in range(cnt.value):
+            name = C.create_string_buffer(64)
+            calls, ms, by = C.c_uint32(), C.c_double(), C.c_double()
+            _check(lib.dmx_profile_get(self._h, i, name, 64, C.byref(calls), C.byref(ms),
+                                       C.byref(by), None))
+            if calls.value:
+                out[name.value.decode()] = {"calls": calls.value, "ms": ms.value,
+                                            "bytes": by.value}
+        return out
 
     def last_run_stats(self):
         ms, la, sw = C.c_double(), C.c_uint32(), C.c_uint32()

@@ -131,6 +131,18 @@ int dmx_set_fusion_max_k(dmx_sim* sim, uint32_t k);
  * sweeps it performed. */
 int dmx_last_run_stats(const dmx_sim* sim, double* ms, uint32_t* launches, uint32_t* sweeps);
 
+/* Extension: time every launch of subsequent runs with a pair of HIP events on
+ * the run's stream (on = 1), or stop (on = 0). */
+int dmx_set_profiling(dmx_sim* sim, int on);
+
+/* Extension: per-kernel-class statistics of the last profiled run.  Class
+ * indices run from 0 to *count - 1 (count may be NULL); name receives the
+ * class name (at most cap bytes with the NUL), calls the launches, ms their
+ * summed event time and bytes their summed algorithmic bytes (entries each
+ * launch must read and write, both planes). */
+int dmx_profile_get(dmx_sim* sim, uint32_t cls, char* name, size_t cap, uint32_t* calls,
+                    double* ms, double* bytes, uint32_t* count);
+
 /* Message of the last failure on this thread ("" if none). */
 const char* dmx_last_error(void);
 
