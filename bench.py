@@ -82,6 +82,54 @@ def cpu_baseline(table, n, gates=120):
     }
 
 
+def batched(steps, warmup, seed=4, n_circuits=256):
+    """The batched configuration: 256 hardware-efficient-ansatz circuits
+    (n uniform in 6..11, depth 10..40, RY/RZ layers + CX ladder), 32 random
+    Pauli strings each; one run = every circuit from |0><0| plus every energy,
+    one graph launch.  Compared with the same circuits run one at a time
+    through the single-circuit plain and fused methods."""
+    import dmsim_amd
+    import workloads as W
+
+    items = W.hea_batch(n_circuits=n_circuits, seed=seed)
+    gates = sum(len(c) for c, _, _ in items)
+    b = dmsim_amd.Batch()
+    for circ, strings, weights in items:
+        b.add_circuit(circ, strings, weights)
+    b.upload()
+    for _ in range(warmup):
+        b.run()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        b.run()
+    wall = (time.perf_counter() - t0) / steps
+    st = b.stats()
+    b.close()
+    out = {
+        "workload": f"{n_circuits} HEA circuits, n in 6..11, depth 10..40, 32 Pauli strings, "
+                    f"seed {seed}",
+        "gates": gates,
+        "batched_gate_updates_per_s": gates / wall,
+        "batched_ms_per_run": wall * 1e3,
+        "batched_device_ms": st["ms"],
+        "launches_per_run": st["launches"],
+    }
+    for mname in ("plain_launch", "fused"):
+        t0 = time.perf_counter()
+        for circ, _, _ in items:
+            s = dmsim_amd.Sim(circ.n_qubits)
+            s.set_method(METHODS[mname])
+            s.append_table(circ)
+            s.upload()
+            s.run()
+            s.close()
+        dt = time.perf_counter() - t0
+        out[f"one_by_one_{mname}_gate_updates_per_s"] = gates / dt
+    out["batched_vs_plain"] = (out["batched_gate_updates_per_s"]
+                               / out["one_by_one_plain_launch_gate_updates_per_s"])
+    return out
+
+
 def calibrate_copy():
     """A plain device-to-device copy of 2 GiB per plane pair (4 GiB read +
     4 GiB written in total) for scaling the FETCH_SIZE/WRITE_SIZE counters."""
@@ -103,6 +151,7 @@ def main():
     ap.add_argument("--n", type=int, default=14)
     ap.add_argument("--methods", default="fused,plain_launch")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--calibrate-copy", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -190,6 +239,11 @@ def main():
             "launches": kv["calls"],
         },
     }
+    if not args.no_batch:
+        log("[bench] batched configuration ...")
+        line["batched"] = batched(args.steps, args.warmup)
+        log(f"[bench] batched: {line['batched']['batched_gate_updates_per_s']:.0f} "
+            f"gate-updates/s, {line['batched']['batched_vs_plain']:.1f}x one-by-one plain")
     if not args.no_cpu_baseline:
         log("[bench] cpu baseline ...")
         line["cpu_baseline"] = cpu_baseline(table, n)

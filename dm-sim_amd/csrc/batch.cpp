@@ -1,0 +1,293 @@
+// batch.cpp -- batched executor (host side).
+//
+// Each circuit is planned into <= 4-qubit blocks by the fused planner.  Run
+// order: reset every rho in the pool; for block step s = 0, 1, ..., one
+// launch per qubit count n covering every circuit of that width that still
+// has a block s (grid.y = circuit); then one workgroup per circuit sums its
+// Pauli terms.  The sequence is captured once into a hipGraph and replayed by
+// each run.
+#include "batch.hpp"
+
+#include <algorithm>
+#include <map>
+
+#include "dmsim_amd.h"
+#include "engine.hpp"
+
+namespace dmx {
+
+#define BTRY(call)                                                                   \
+    do {                                                                             \
+        hipError_t e_ = (call);                                                      \
+        if (e_ != hipSuccess) {                                                      \
+            set_error(std::string(#call) + ": " + hipGetErrorString(e_));            \
+            return e_ == hipErrorOutOfMemory ? DMX_ENOMEM : DMX_EHIP;                \
+        }                                                                            \
+    } while (0)
+
+Batch::~Batch()
+{
+    free_device();
+    if (ev0_) (void)hipEventDestroy(ev0_);
+    if (ev1_) (void)hipEventDestroy(ev1_);
+    if (stream_) (void)hipStreamDestroy(stream_);
+}
+
+void Batch::free_device()
+{
+    if (exec_) (void)hipGraphExecDestroy(exec_);
+    if (graph_) (void)hipGraphDestroy(graph_);
+    exec_ = nullptr;
+    graph_ = nullptr;
+    for (void* p : {(void*)pool_, (void*)d_blocks_, (void*)d_entries_, (void*)d_re_off_,
+                    (void*)d_terms_, (void*)d_jobs_, (void*)d_energy_})
+        if (p) (void)hipFree(p);
+    pool_ = nullptr;
+    d_blocks_ = nullptr;
+    d_entries_ = nullptr;
+    d_re_off_ = nullptr;
+    d_terms_ = nullptr;
+    d_jobs_ = nullptr;
+    d_energy_ = nullptr;
+}
+
+int Batch::init()
+{
+    BTRY(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+    BTRY(hipEventCreate(&ev0_));
+    BTRY(hipEventCreate(&ev1_));
+    return DMX_OK;
+}
+
+uint64_t Batch::gates() const
+{
+    uint64_t g = 0;
+    for (const Circ& c : circs_) g += c.gates.size();
+    return g;
+}
+
+int Batch::add_circuit(uint32_t n, std::vector<Gate> gates, uint32_t* id)
+{
+    if (uploaded_) {
+        set_error("add_circuit after upload");
+        return DMX_ESTATE;
+    }
+    if (n < 4 || n > 14) {
+        set_error("batched circuits need 4 <= n_qubits <= 14");
+        return DMX_EINVAL;
+    }
+    for (const Gate& g : gates) {
+        if (g.op < 0 || g.op >= OP_MAT) {
+            set_error("unknown op code " + std::to_string(g.op));
+            return DMX_EINVAL;
+        }
+        const int k = op_arity(g.op);
+        for (int i = 0; i < k; i++) {
+            if ((int)g.qb[i] >= (int)n) {
+                set_error("qubit index out of range in a batched circuit");
+                return DMX_EINVAL;
+            }
+            for (int j2 = 0; j2 < i; j2++)
+                if (g.qb[i] == g.qb[j2]) {
+                    set_error("operands must be distinct qubits");
+                    return DMX_EINVAL;
+                }
+        }
+    }
+    Circ c;
+    c.n = (int)n;
+    c.gates = std::move(gates);
+    circs_.push_back(std::move(c));
+    if (id) *id = (uint32_t)(circs_.size() - 1);
+    return DMX_OK;
+}
+
+int Batch::add_pauli(uint32_t id, const char* pauli, double weight)
+{
+    if (uploaded_) {
+        set_error("add_pauli after upload");
+        return DMX_ESTATE;
+    }
+    if (id >= circs_.size() || !pauli) {
+        set_error("bad circuit id or null Pauli string");
+        return DMX_EINVAL;
+    }
+    Circ& c = circs_[id];
+    const std::string s(pauli);
+    if ((int)s.size() != c.n) {
+        set_error("Pauli string length must equal the circuit's qubit count");
+        return DMX_EINVAL;
+    }
+    PauliTerm t{0, 0, 0, 0, weight};
+    for (int q = 0; q < c.n; q++) {
+        switch (s[q]) {
+        case 'I': break;
+        case 'X': t.xmask |= 1ull << q; break;
+        case 'Y': t.xmask |= 1ull << q; t.zymask |= 1ull << q; t.ny++; break;
+        case 'Z': t.zymask |= 1ull << q; break;
+        default:
+            set_error("Pauli string characters must be I, X, Y or Z");
+            return DMX_EINVAL;
+        }
+    }
+    c.paulis.push_back(t);
+    return DMX_OK;
+}
+
+int Batch::upload()
+{
+    if (uploaded_) {
+        set_error("batch already uploaded");
+        return DMX_ESTATE;
+    }
+    if (circs_.empty() || circs_.size() > 65535) {
+        set_error("a batch needs 1..65535 circuits");
+        return DMX_EINVAL;
+    }
+    std::vector<FusedBlock> blocks;
+    std::vector<PauliTerm> terms;
+    std::vector<uint64_t> re_off;
+    uint64_t off = 0;
+    size_t max_steps = 0;
+    for (Circ& c : circs_) {
+        BlockPlan bp;
+        bool fusable = false;
+        int rc = plan_blocks(c.gates, c.n, 4, bp, fusable);
+        if (rc) return rc;
+        if (!fusable) {
+            set_error("batched circuits must be fusable (no SRN)");
+            return DMX_EINVAL;
+        }
+        for (const auto& o : bp.order)
+            if (o.first != 0) {
+                set_error("batched circuits take gates on at most 4 qubits");
+                return DMX_EINVAL;
+            }
+        c.blk_off = blocks.size();
+        c.n_blocks = bp.blocks.size();
+        blocks.insert(blocks.end(), bp.blocks.begin(), bp.blocks.end());
+        c.re_off = off;
+        c.im_off = off + (1ull << (2 * c.n));
+        off += 2ull << (2 * c.n);
+        re_off.push_back(c.re_off);
+        max_steps = std::max(max_steps, c.n_blocks);
+    }
+    pool_doubles_ = off;
+    BTRY(hipMalloc(&pool_, pool_doubles_ * sizeof(double)));
+    if (!blocks.empty()) BTRY(hipMalloc(&d_blocks_, blocks.size() * sizeof(FusedBlock)));
+    BTRY(hipMalloc(&d_re_off_, re_off.size() * sizeof(uint64_t)));
+    BTRY(hipMalloc(&d_energy_, circs_.size() * sizeof(double)));
+
+    // launches: per block step, one per qubit count
+    std::vector<BatchEntry> entries;
+    std::map<int, std::vector<int>> by_n;
+    for (int i = 0; i < (int)circs_.size(); i++) by_n[circs_[i].n].push_back(i);
+    for (size_t s = 0; s < max_steps; s++)
+        for (const auto& kv : by_n) {
+            Launch l{kv.first, 0, entries.size()};
+            for (int i : kv.second) {
+                const Circ& c = circs_[i];
+                if (s >= c.n_blocks) continue;
+                entries.push_back({pool_ + c.re_off, pool_ + c.im_off, d_blocks_ + c.blk_off + s});
+                l.count++;
+            }
+            if (l.count) launches_.push_back(l);
+        }
+    if (!entries.empty()) BTRY(hipMalloc(&d_entries_, entries.size() * sizeof(BatchEntry)));
+
+    std::vector<EnergyJob> jobs;
+    for (const Circ& c : circs_) {
+        jobs.push_back({pool_ + c.re_off, pool_ + c.im_off, c.n, (int32_t)terms.size(),
+                        (int32_t)c.paulis.size(), 0});
+        terms.insert(terms.end(), c.paulis.begin(), c.paulis.end());
+    }
+    if (terms.empty()) terms.push_back(PauliTerm{0, 0, 0, 0, 0.0});  // keep a valid pointer
+    BTRY(hipMalloc(&d_terms_, terms.size() * sizeof(PauliTerm)));
+    BTRY(hipMalloc(&d_jobs_, jobs.size() * sizeof(EnergyJob)));
+
+    if (!blocks.empty())
+        BTRY(hipMemcpy(d_blocks_, blocks.data(), blocks.size() * sizeof(FusedBlock),
+                       hipMemcpyHostToDevice));
+    if (!entries.empty())
+        BTRY(hipMemcpy(d_entries_, entries.data(), entries.size() * sizeof(BatchEntry),
+                       hipMemcpyHostToDevice));
+    BTRY(hipMemcpy(d_re_off_, re_off.data(), re_off.size() * sizeof(uint64_t),
+                   hipMemcpyHostToDevice));
+    BTRY(hipMemcpy(d_terms_, terms.data(), terms.size() * sizeof(PauliTerm),
+                   hipMemcpyHostToDevice));
+    BTRY(hipMemcpy(d_jobs_, jobs.data(), jobs.size() * sizeof(EnergyJob), hipMemcpyHostToDevice));
+
+    // capture the run once; replays reset the pool themselves
+    BTRY(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
+    const int rc = enqueue();
+    hipGraph_t g = nullptr;
+    const hipError_t ec = hipStreamEndCapture(stream_, &g);
+    if (rc) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc;
+    }
+    BTRY(ec);
+    graph_ = g;
+    BTRY(hipGraphInstantiate(&exec_, graph_, nullptr, nullptr, 0));
+    energy_.assign(circs_.size(), 0.0);
+    uploaded_ = true;
+    return DMX_OK;
+}
+
+int Batch::enqueue()
+{
+    BTRY(launch_pool_reset(pool_, pool_doubles_, d_re_off_, (int)circs_.size(), stream_));
+    for (const Launch& l : launches_)
+        BTRY(launch_fused4_batch(d_entries_ + l.entry_off, l.count, l.n, 4096, stream_));
+    BTRY(launch_pauli_energy(d_jobs_, d_terms_, (int)circs_.size(), d_energy_, stream_));
+    return DMX_OK;
+}
+
+int Batch::run()
+{
+    if (!uploaded_) {
+        set_error("batch run before upload");
+        return DMX_ESTATE;
+    }
+    BTRY(hipEventRecord(ev0_, stream_));
+    if (use_graph_) {
+        BTRY(hipGraphLaunch(exec_, stream_));
+    } else {
+        const int rc = enqueue();
+        if (rc) return rc;
+    }
+    BTRY(hipEventRecord(ev1_, stream_));
+    BTRY(hipEventSynchronize(ev1_));
+    float ms = 0.f;
+    BTRY(hipEventElapsedTime(&ms, ev0_, ev1_));
+    last_ms_ = ms;
+    BTRY(hipMemcpy(energy_.data(), d_energy_, energy_.size() * sizeof(double),
+                   hipMemcpyDeviceToHost));
+    ran_ = true;
+    return DMX_OK;
+}
+
+int Batch::energies(double* out) const
+{
+    if (!ran_) {
+        set_error("batch energies before run");
+        return DMX_ESTATE;
+    }
+    std::copy(energy_.begin(), energy_.end(), out);
+    return DMX_OK;
+}
+
+int Batch::get_dm(uint32_t id, double* re, double* im)
+{
+    if (!ran_ || id >= circs_.size()) {
+        set_error(!ran_ ? "batch get_dm before run" : "bad circuit id");
+        return !ran_ ? DMX_ESTATE : DMX_EINVAL;
+    }
+    const Circ& c = circs_[id];
+    const size_t bytes = (1ull << (2 * c.n)) * sizeof(double);
+    if (re) BTRY(hipMemcpy(re, pool_ + c.re_off, bytes, hipMemcpyDeviceToHost));
+    if (im) BTRY(hipMemcpy(im, pool_ + c.im_off, bytes, hipMemcpyDeviceToHost));
+    return DMX_OK;
+}
+
+}  // namespace dmx

@@ -1,0 +1,99 @@
+// batch.hpp -- batched executor: many circuits of uneven width and depth, all
+// rho resident in one pool, one graph launch per run.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "fuse.hpp"
+#include "fused.hpp"
+#include "lower.hpp"
+
+namespace dmx {
+
+struct PauliTerm {
+    uint64_t xmask;   // qubits carrying X or Y
+    uint64_t zymask;  // qubits carrying Z or Y
+    int32_t ny;       // number of Y factors
+    int32_t pad;
+    double weight;
+};
+
+struct EnergyJob {
+    const double* re;
+    const double* im;
+    int32_t n;
+    int32_t first_term;
+    int32_t n_terms;
+    int32_t pad;
+};
+
+hipError_t launch_pool_reset(double* pool, uint64_t n_doubles, const uint64_t* d_re_off,
+                             int count, hipStream_t stream);
+hipError_t launch_pauli_energy(const EnergyJob* d_jobs, const PauliTerm* d_terms, int count,
+                               double* d_out, hipStream_t stream);
+
+class Batch {
+public:
+    Batch() = default;
+    ~Batch();
+    Batch(const Batch&) = delete;
+    Batch& operator=(const Batch&) = delete;
+
+    int init();
+    int add_circuit(uint32_t n, std::vector<Gate> gates, uint32_t* id);
+    int add_pauli(uint32_t id, const char* pauli, double weight);
+    int upload();
+    int run();
+    int energies(double* out) const;
+    int get_dm(uint32_t id, double* re, double* im);
+    void set_graph(bool on) { use_graph_ = on; }
+
+    double last_ms() const { return last_ms_; }
+    uint32_t launches() const { return (uint32_t)launches_.size() + 3; }
+    uint32_t circuits() const { return (uint32_t)circs_.size(); }
+    uint64_t gates() const;
+
+private:
+    struct Circ {
+        int n;
+        std::vector<Gate> gates;
+        std::vector<PauliTerm> paulis;
+        uint64_t re_off = 0, im_off = 0;
+        size_t blk_off = 0;
+        size_t n_blocks = 0;
+    };
+    struct Launch {
+        int n;
+        int count;
+        size_t entry_off;
+    };
+    int enqueue();  // the whole run on stream_ (captured or eager)
+    void free_device();
+
+    std::vector<Circ> circs_;
+    std::vector<Launch> launches_;
+    bool uploaded_ = false;
+    bool use_graph_ = true;
+    bool ran_ = false;
+
+    double* pool_ = nullptr;
+    uint64_t pool_doubles_ = 0;
+    FusedBlock* d_blocks_ = nullptr;
+    BatchEntry* d_entries_ = nullptr;
+    uint64_t* d_re_off_ = nullptr;
+    PauliTerm* d_terms_ = nullptr;
+    EnergyJob* d_jobs_ = nullptr;
+    double* d_energy_ = nullptr;
+    std::vector<double> energy_;
+
+    hipStream_t stream_ = nullptr;
+    hipEvent_t ev0_ = nullptr, ev1_ = nullptr;
+    hipGraph_t graph_ = nullptr;
+    hipGraphExec_t exec_ = nullptr;
+    double last_ms_ = 0.0;
+};
+
+}  // namespace dmx

@@ -3,12 +3,18 @@
 #include <exception>
 #include <new>
 #include <string>
+#include <vector>
 
 #include "dmsim_amd.h"
+#include "batch.hpp"
 #include "engine.hpp"
 
 struct dmx_sim {
     dmx::Engine eng;
+};
+
+struct dmx_batch {
+    dmx::Batch b;
 };
 
 namespace {
@@ -244,6 +250,104 @@ int dmx_profile_get(dmx_sim* sim, uint32_t cls, char* name, size_t cap, uint32_t
     if (calls) *calls = st.calls;
     if (ms) *ms = st.ms;
     if (bytes) *bytes = st.bytes;
+    return DMX_OK;
+}
+
+int dmx_batch_create(dmx_batch** out)
+{
+    if (!out) {
+        dmx::set_error("null output pointer");
+        return DMX_EINVAL;
+    }
+    *out = nullptr;
+    return guarded([&] {
+        dmx_batch* b = new dmx_batch();
+        int rc = b->b.init();
+        if (rc) {
+            delete b;
+            return rc;
+        }
+        *out = b;
+        return DMX_OK;
+    });
+}
+
+int dmx_batch_destroy(dmx_batch* b)
+{
+    delete b;
+    return DMX_OK;
+}
+
+int dmx_batch_add_circuit(dmx_batch* b, uint32_t n_qubits, uint32_t n_gates, const int32_t* ops,
+                          const uint32_t* qb, const double* params, uint32_t* id)
+{
+    if (!b) return null_handle();
+    if (n_gates > 0 && (!ops || !qb || !params)) {
+        dmx::set_error("null gate table");
+        return DMX_EINVAL;
+    }
+    return guarded([&] {
+        std::vector<dmx::Gate> gates(n_gates);
+        for (uint32_t i = 0; i < n_gates; i++) {
+            gates[i].op = ops[i];
+            std::memcpy(gates[i].qb, qb + 5 * (size_t)i, sizeof(gates[i].qb));
+            gates[i].theta = params[3 * (size_t)i];
+            gates[i].phi = params[3 * (size_t)i + 1];
+            gates[i].lambda = params[3 * (size_t)i + 2];
+        }
+        return b->b.add_circuit(n_qubits, std::move(gates), id);
+    });
+}
+
+int dmx_batch_add_pauli(dmx_batch* b, uint32_t id, const char* pauli, double weight)
+{
+    if (!b) return null_handle();
+    return guarded([&] { return b->b.add_pauli(id, pauli, weight); });
+}
+
+int dmx_batch_upload(dmx_batch* b)
+{
+    if (!b) return null_handle();
+    return guarded([&] { return b->b.upload(); });
+}
+
+int dmx_batch_set_graph(dmx_batch* b, int on)
+{
+    if (!b) return null_handle();
+    b->b.set_graph(on != 0);
+    return DMX_OK;
+}
+
+int dmx_batch_run(dmx_batch* b)
+{
+    if (!b) return null_handle();
+    return guarded([&] { return b->b.run(); });
+}
+
+int dmx_batch_energies(dmx_batch* b, double* out)
+{
+    if (!b) return null_handle();
+    if (!out) {
+        dmx::set_error("null output array");
+        return DMX_EINVAL;
+    }
+    return guarded([&] { return b->b.energies(out); });
+}
+
+int dmx_batch_get_dm(dmx_batch* b, uint32_t id, double* re, double* im)
+{
+    if (!b) return null_handle();
+    return guarded([&] { return b->b.get_dm(id, re, im); });
+}
+
+int dmx_batch_stats(const dmx_batch* b, double* ms, uint32_t* launches, uint32_t* circuits,
+                    uint64_t* gates)
+{
+    if (!b) return null_handle();
+    if (ms) *ms = b->b.last_ms();
+    if (launches) *launches = b->b.launches();
+    if (circuits) *circuits = b->b.circuits();
+    if (gates) *gates = b->b.gates();
     return DMX_OK;
 }
 

@@ -153,9 +153,10 @@ void apply_embedded(std::vector<cd>& U, uint32_t qmask, const std::vector<uint32
 
 }  // namespace
 
-int build_fused_plan(const std::vector<Gate>& circuit, int n, int max_k, FusedPlan** out)
+int plan_blocks(const std::vector<Gate>& circuit, int n, int max_k, BlockPlan& out, bool& fusable)
 {
-    *out = nullptr;
+    out = BlockPlan{};
+    fusable = false;
     if (n < 4) return DMX_OK;  // too few qubits for a 4-qubit block: plain sweeps
     const int kmax = std::min(std::max(max_k, 1), 4);
     for (const Gate& g : circuit) {
@@ -165,13 +166,13 @@ int build_fused_plan(const std::vector<Gate>& circuit, int n, int max_k, FusedPl
             return DMX_EINVAL;
         }
     }
-    // planning: blocks[frozen..] may still take gates; a wide gate freezes all
+    fusable = true;
+    // blocks[frozen..] may still take gates; a gate wider than 4 qubits freezes all
     struct PB {
         uint32_t mask;
         std::vector<int> gates;
     };
     std::vector<PB> pbs;
-    std::vector<std::pair<int, int>> order;  // (kind 0 = block / 1 = wide gate, index)
     int frozen = 0;
     for (int gi = 0; gi < (int)circuit.size(); gi++) {
         const Gate& g = circuit[gi];
@@ -179,7 +180,7 @@ int build_fused_plan(const std::vector<Gate>& circuit, int n, int max_k, FusedPl
         const uint32_t S = gate_mask(g);
         const int k_here = __builtin_popcount(S);
         if (k_here > 4) {
-            order.push_back({1, gi});
+            out.order.push_back({1, gi});
             frozen = (int)pbs.size();
             continue;
         }
@@ -201,20 +202,14 @@ int build_fused_plan(const std::vector<Gate>& circuit, int n, int max_k, FusedPl
             pbs[best].mask |= S;
             pbs[best].gates.push_back(gi);
         } else {
-            order.push_back({0, (int)pbs.size()});
+            out.order.push_back({0, (int)pbs.size()});
             pbs.push_back({S, {gi}});
         }
     }
-
-    Plan* plan = new Plan();
-    plan->n = n;
     const uint32_t all = (1u << n) - 1;
     for (const PB& pb : pbs) {
         uint32_t qmask = pb.mask;
         for (int q = 0; q < n && __builtin_popcount(qmask) < 4; q++) qmask |= 1u << q;
-        uint32_t cmask = qmask;
-        const int m = std::min(8, n);
-        for (int q = 0; q < n && __builtin_popcount(cmask) < m; q++) cmask |= 1u << q;
         std::vector<cd> U(256, 0.0);
         for (int i = 0; i < 16; i++) U[(size_t)i * 16 + i] = 1.0;
         for (int gi : pb.gates) {
@@ -227,15 +222,28 @@ int build_fused_plan(const std::vector<Gate>& circuit, int n, int max_k, FusedPl
         FusedBlock fb;
         std::memset(&fb, 0, sizeof(fb));
         fb.qmask = qmask & all;
-        fb.cmask = cmask & all;
         for (int i = 0; i < 256; i++) {
             fb.ur[i] = U[i].real();
             fb.ui[i] = U[i].imag();
         }
-        plan->blocks.push_back(fb);
-        plan->gates_per_block.push_back((uint32_t)pb.gates.size());
+        out.blocks.push_back(fb);
+        out.gates_per_block.push_back((uint32_t)pb.gates.size());
     }
-    for (const auto& o : order) {
+    return DMX_OK;
+}
+
+int build_fused_plan(const std::vector<Gate>& circuit, int n, int max_k, FusedPlan** out)
+{
+    *out = nullptr;
+    BlockPlan bp;
+    bool fusable = false;
+    int rc = plan_blocks(circuit, n, max_k, bp, fusable);
+    if (rc || !fusable) return rc;
+    Plan* plan = new Plan();
+    plan->n = n;
+    plan->blocks = std::move(bp.blocks);
+    plan->gates_per_block = std::move(bp.gates_per_block);
+    for (const auto& o : bp.order) {
         if (o.first == 0) {
             plan->steps.push_back({true, o.second, 0});
         } else {

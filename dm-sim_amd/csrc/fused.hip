@@ -50,14 +50,15 @@ __device__ __forceinline__ uint64_t deposit(uint64_t v, uint64_t mask)
 
 }  // namespace
 
-// M = number of super-tile column bits (8 for n >= 8, n below that).
+// Super-tiles st_first, st_first + st_stride, ... of one block sweep; s_re and
+// s_im are the workgroup's 16 x kPitch LDS planes.  M = number of super-tile
+// column bits (8 for n >= 8, n below that).
 template <int M>
-__global__ void __launch_bounds__(256, 2) k_fused4(double* __restrict__ re, double* __restrict__ im,
-                                                    const FusedBlock* __restrict__ blk, int n)
+__device__ __forceinline__ void fused_sweep(double* __restrict__ re, double* __restrict__ im,
+                                            const FusedBlock* __restrict__ blk, int n,
+                                            uint64_t st_first, uint64_t st_stride,
+                                            double* __restrict__ s_re, double* __restrict__ s_im)
 {
-    __shared__ double s_re[16 * kPitch];
-    __shared__ double s_im[16 * kPitch];
-
     const int tid = threadIdx.x;
     const int lane = tid & 63;
     const int wave = tid >> 6;
@@ -128,7 +129,7 @@ __global__ void __launch_bounds__(256, 2) k_fused4(double* __restrict__ re, doub
         return deposit(rg, row_free) * dim + deposit(cg, col_free) + coloff;
     };
 
-    for (uint64_t st = blockIdx.x; st < n_st; st += gridDim.x) {
+    for (uint64_t st = st_first; st < n_st; st += st_stride) {
         const uint64_t g0 = base_of(st);
         if (active) {
             double2 vr[iters], vi[iters];
@@ -201,19 +202,66 @@ __global__ void __launch_bounds__(256, 2) k_fused4(double* __restrict__ re, doub
     }
 }
 
+template <int M>
+__global__ void __launch_bounds__(256, 2) k_fused4(double* __restrict__ re, double* __restrict__ im,
+                                                    const FusedBlock* __restrict__ blk, int n)
+{
+    __shared__ double s_re[16 * kPitch];
+    __shared__ double s_im[16 * kPitch];
+    fused_sweep<M>(re, im, blk, n, blockIdx.x, gridDim.x, s_re, s_im);
+}
+
+// Batched form: blockIdx.y selects one circuit's (rho, block) pair, all of
+// qubit count n.
+template <int M>
+__global__ void __launch_bounds__(256, 2) k_fused4_batch(const BatchEntry* __restrict__ entries,
+                                                          int n)
+{
+    __shared__ double s_re[16 * kPitch];
+    __shared__ double s_im[16 * kPitch];
+    const BatchEntry e = entries[blockIdx.y];
+    fused_sweep<M>(e.re, e.im, e.blk, n, blockIdx.x, gridDim.x, s_re, s_im);
+}
+
+namespace {
+
+uint64_t supertiles(int n)
+{
+    const int m = n < 8 ? n : 8;
+    return (1ull << (n - 4)) * (1ull << (n - m));
+}
+
+}  // namespace
+
 hipError_t launch_fused4(double* re, double* im, const FusedBlock* d_blk, int n, int max_blocks,
                          hipStream_t stream)
 {
     if (n < 4) return hipErrorInvalidValue;
-    const int m = n < 8 ? n : 8;
-    const uint64_t n_st = (1ull << (n - 4)) * (1ull << (n - m));
+    const uint64_t n_st = supertiles(n);
     const unsigned blocks = (unsigned)(n_st < (uint64_t)max_blocks ? n_st : (uint64_t)max_blocks);
-    switch (m) {
+    switch (n < 8 ? n : 8) {
     case 4: k_fused4<4><<<blocks, 256, 0, stream>>>(re, im, d_blk, n); break;
     case 5: k_fused4<5><<<blocks, 256, 0, stream>>>(re, im, d_blk, n); break;
     case 6: k_fused4<6><<<blocks, 256, 0, stream>>>(re, im, d_blk, n); break;
     case 7: k_fused4<7><<<blocks, 256, 0, stream>>>(re, im, d_blk, n); break;
     default: k_fused4<8><<<blocks, 256, 0, stream>>>(re, im, d_blk, n); break;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_fused4_batch(const BatchEntry* d_entries, int count, int n, int max_blocks,
+                               hipStream_t stream)
+{
+    if (n < 4 || count < 1 || count > 65535) return hipErrorInvalidValue;
+    const uint64_t n_st = supertiles(n);
+    const dim3 grid((unsigned)(n_st < (uint64_t)max_blocks ? n_st : (uint64_t)max_blocks),
+                    (unsigned)count);
+    switch (n < 8 ? n : 8) {
+    case 4: k_fused4_batch<4><<<grid, 256, 0, stream>>>(d_entries, n); break;
+    case 5: k_fused4_batch<5><<<grid, 256, 0, stream>>>(d_entries, n); break;
+    case 6: k_fused4_batch<6><<<grid, 256, 0, stream>>>(d_entries, n); break;
+    case 7: k_fused4_batch<7><<<grid, 256, 0, stream>>>(d_entries, n); break;
+    default: k_fused4_batch<8><<<grid, 256, 0, stream>>>(d_entries, n); break;
     }
     return hipGetLastError();
 }

@@ -24,7 +24,9 @@ EXPORTS = [
     "dmx_clear_circuit", "dmx_reset", "dmx_reset_dm", "dmx_measure", "dmx_get_dm",
     "dmx_get_diag", "dmx_expect_z", "dmx_info", "dmx_dump", "dmx_set_method",
     "dmx_set_fusion_max_k", "dmx_last_run_stats", "dmx_set_profiling", "dmx_profile_get",
-    "dmx_last_error",
+    "dmx_batch_create", "dmx_batch_destroy", "dmx_batch_add_circuit", "dmx_batch_add_pauli",
+    "dmx_batch_upload", "dmx_batch_set_graph", "dmx_batch_run", "dmx_batch_energies",
+    "dmx_batch_get_dm", "dmx_batch_stats", "dmx_last_error",
 ]
 
 _lib = None
@@ -65,6 +67,18 @@ def load():
     lib.dmx_set_profiling.argtypes = [vp, C.c_int]
     lib.dmx_profile_get.argtypes = [vp, C.c_uint32, C.c_char_p, C.c_size_t, P(C.c_uint32),
                                     P(C.c_double), P(C.c_double), P(C.c_uint32)]
+    lib.dmx_batch_create.argtypes = [P(vp)]
+    lib.dmx_batch_destroy.argtypes = [vp]
+    lib.dmx_batch_add_circuit.argtypes = [vp, C.c_uint32, C.c_uint32, P(C.c_int32),
+                                          P(C.c_uint32), P(C.c_double), P(C.c_uint32)]
+    lib.dmx_batch_add_pauli.argtypes = [vp, C.c_uint32, C.c_char_p, C.c_double]
+    for name in ("dmx_batch_upload", "dmx_batch_run"):
+        getattr(lib, name).argtypes = [vp]
+    lib.dmx_batch_set_graph.argtypes = [vp, C.c_int]
+    lib.dmx_batch_energies.argtypes = [vp, P(C.c_double)]
+    lib.dmx_batch_get_dm.argtypes = [vp, C.c_uint32, P(C.c_double), P(C.c_double)]
+    lib.dmx_batch_stats.argtypes = [vp, P(C.c_double), P(C.c_uint32), P(C.c_uint32),
+                                    P(C.c_uint64)]
     lib.dmx_last_error.restype = C.c_char_p
     for name in EXPORTS:
         if name != "dmx_last_error":
@@ -218,3 +232,64 @@ class Sim:
         ms, la, sw = C.c_double(), C.c_uint32(), C.c_uint32()
         _check(load().dmx_last_run_stats(self._h, C.byref(ms), C.byref(la), C.byref(sw)))
         return {"ms": ms.value, "launches": la.value, "sweeps": sw.value}
+
+
+class Batch:
+    """Many independent circuits run together (``dmx_batch_*``)."""
+
+    def __init__(self):
+        lib = load()
+        h = C.c_void_p()
+        _check(lib.dmx_batch_create(C.byref(h)))
+        self._h = h
+        self.sizes = []
+
+    def close(self):
+        if getattr(self, "_h", None):
+            load().dmx_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def add_circuit(self, table, paulis=(), weights=()):
+        """A workloads.GateTable (ops 0..37) plus its Pauli terms; returns the id."""
+        ops, qb, prm, _ = table.arrays()
+        cid = C.c_uint32()
+        _check(load().dmx_batch_add_circuit(self._h, table.n_qubits, len(ops),
+                                            _ptr(ops, C.c_int32), _ptr(qb, C.c_uint32),
+                                            _ptr(prm, C.c_double), C.byref(cid)))
+        for p, w in zip(paulis, weights):
+            _check(load().dmx_batch_add_pauli(self._h, cid.value, p.encode(), float(w)))
+        self.sizes.append(table.n_qubits)
+        return cid.value
+
+    def upload(self):
+        _check(load().dmx_batch_upload(self._h))
+
+    def set_graph(self, on: bool):
+        _check(load().dmx_batch_set_graph(self._h, 1 if on else 0))
+
+    def run(self):
+        _check(load().dmx_batch_run(self._h))
+
+    def energies(self):
+        out = np.empty(len(self.sizes))
+        _check(load().dmx_batch_energies(self._h, _ptr(out, C.c_double)))
+        return out
+
+    def get_dm(self, cid):
+        d = 1 << self.sizes[cid]
+        re = np.empty(d * d)
+        im = np.empty(d * d)
+        _check(load().dmx_batch_get_dm(self._h, cid, _ptr(re, C.c_double), _ptr(im, C.c_double)))
+        return re, im
+
+    def stats(self):
+        ms, la, nc, ng = C.c_double(), C.c_uint32(), C.c_uint32(), C.c_uint64()
+        _check(load().dmx_batch_stats(self._h, C.byref(ms), C.byref(la), C.byref(nc),
+                                      C.byref(ng)))
+        return {"ms": ms.value, "launches": la.value, "circuits": nc.value, "gates": ng.value}

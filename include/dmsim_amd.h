@@ -143,6 +143,43 @@ int dmx_set_profiling(dmx_sim* sim, int on);
 int dmx_profile_get(dmx_sim* sim, uint32_t cls, char* name, size_t cap, uint32_t* calls,
                     double* ms, double* bytes, uint32_t* count);
 
+/* ---- Batched executor.  Extension with no counterpart in the reference: many
+ * independent circuits of uneven width (4..14 qubits) and depth on one GPU,
+ * every rho resident in one device pool, each run a single hipGraph launch
+ * (reset, fused block sweeps, Pauli-string energies). ---- */
+typedef struct dmx_batch dmx_batch;
+
+int dmx_batch_create(dmx_batch** out);
+int dmx_batch_destroy(dmx_batch* b);
+
+/* One circuit as a gate table (ops 0..37 as in dmx_append; qb[5*i..], params
+ * [3*i..] = theta, phi, lambda); *id receives its index. */
+int dmx_batch_add_circuit(dmx_batch* b, uint32_t n_qubits, uint32_t n_gates, const int32_t* ops,
+                          const uint32_t* qb, const double* params, uint32_t* id);
+
+/* A Hamiltonian term of circuit id: pauli = n characters from "IXYZ", qubit 0
+ * first, with a real weight. */
+int dmx_batch_add_pauli(dmx_batch* b, uint32_t id, const char* pauli, double weight);
+
+/* Plans every circuit, allocates the pool and captures the run into a graph. */
+int dmx_batch_upload(dmx_batch* b);
+
+/* Replay the captured graph (on = 1, default) or enqueue the launches eagerly. */
+int dmx_batch_set_graph(dmx_batch* b, int on);
+
+/* Every circuit from |0..0><0..0|; energies become available. */
+int dmx_batch_run(dmx_batch* b);
+
+/* E_c = sum of weight * Re Tr(P rho_c) per circuit, out[circuit count]. */
+int dmx_batch_energies(dmx_batch* b, double* out);
+
+/* rho of circuit id after the last run, dmx_get_dm's convention (4^n doubles each). */
+int dmx_batch_get_dm(dmx_batch* b, uint32_t id, double* re, double* im);
+
+/* Device time of the last run, launches per run, circuits, total API gates. */
+int dmx_batch_stats(const dmx_batch* b, double* ms, uint32_t* launches, uint32_t* circuits,
+                    uint64_t* gates);
+
 /* Message of the last failure on this thread ("" if none). */
 const char* dmx_last_error(void);
 
