@@ -14,7 +14,7 @@ launch over its mean launch time, both from HIP events recorded around every
 launch on the run's stream during the timed steps.  `cpu_baseline` times the
 oracle's CPU restatement of the reference path on a bounded sample.
 
-  python bench.py                      # default: fused and plain, n = 14
+  python bench.py                      # default: fused (greedy and learned plans), plain
   python bench.py --calibrate-copy     # one 2 x 2 GiB device copy (counter scaling)
 """
 import argparse
@@ -33,7 +33,7 @@ HBM_PEAK_GBS = 8000.0
 # WRITE_SIZE runs of this workload scaled by a copy of known size
 # (profiles/k_fused4_n14_hbm_counters.md).
 TRAFFIC_RATIO = {"k_fused4": 8.591e9 / 8.590e9}
-METHODS = {"plain": 0, "plain_launch": 1, "fused": 2}
+METHODS = {"plain": 0, "plain_launch": 1, "fused": 2, "fused_learned": 2}
 
 
 def log(msg):
@@ -86,47 +86,64 @@ def batched(steps, warmup, seed=4, n_circuits=256):
     """The batched configuration: 256 hardware-efficient-ansatz circuits
     (n uniform in 6..11, depth 10..40, RY/RZ layers + CX ladder), 32 random
     Pauli strings each; one run = every circuit from |0><0| plus every energy,
-    one graph launch.  Compared with the same circuits run one at a time
-    through the single-circuit plain and fused methods."""
+    one graph launch.  Timed with the autotuned tile widths and with the
+    default widths, and compared with the same circuits run one after another
+    (each already uploaded; only its reset and run are timed) through the
+    single-circuit plain and fused methods."""
     import dmsim_amd
     import workloads as W
 
     items = W.hea_batch(n_circuits=n_circuits, seed=seed)
     gates = sum(len(c) for c, _, _ in items)
-    b = dmsim_amd.Batch()
-    for circ, strings, weights in items:
-        b.add_circuit(circ, strings, weights)
-    b.upload()
-    for _ in range(warmup):
-        b.run()
-    t0 = time.perf_counter()
-    for _ in range(steps):
-        b.run()
-    wall = (time.perf_counter() - t0) / steps
-    st = b.stats()
-    b.close()
     out = {
         "workload": f"{n_circuits} HEA circuits, n in 6..11, depth 10..40, 32 Pauli strings, "
                     f"seed {seed}",
         "gates": gates,
-        "batched_gate_updates_per_s": gates / wall,
-        "batched_ms_per_run": wall * 1e3,
-        "batched_device_ms": st["ms"],
-        "launches_per_run": st["launches"],
     }
-    for mname in ("plain_launch", "fused"):
+    for tuned in (True, False):
+        b = dmsim_amd.Batch()
+        b.set_autotune(tuned)
+        for circ, strings, weights in items:
+            b.add_circuit(circ, strings, weights)
+        b.upload()
+        for _ in range(warmup):
+            b.run()
         t0 = time.perf_counter()
+        for _ in range(steps):
+            b.run()
+        wall = (time.perf_counter() - t0) / steps
+        st = b.stats()
+        key = "batched" if tuned else "batched_default_tiles"
+        out[f"{key}_gate_updates_per_s"] = gates / wall
+        out[f"{key}_ms_per_run"] = wall * 1e3
+        out[f"{key}_device_ms"] = st["ms"]
+        if tuned:
+            out["launches_per_run"] = st["launches"]
+            out["tile_bits"] = {n: b.tile_bits(n) for n in range(6, 12)}
+        b.close()
+    for mname in ("plain_launch", "fused"):
+        sims = {}
+        dt = 0.0
         for circ, _, _ in items:
-            s = dmsim_amd.Sim(circ.n_qubits)
-            s.set_method(METHODS[mname])
+            n = circ.n_qubits
+            if n not in sims:
+                sims[n] = dmsim_amd.Sim(n)
+                sims[n].set_method(METHODS[mname])
+            s = sims[n]
+            s.clear_circuit()
             s.append_table(circ)
             s.upload()
+            t0 = time.perf_counter()
+            s.reset_dm()
             s.run()
+            dt += time.perf_counter() - t0
+        for s in sims.values():
             s.close()
-        dt = time.perf_counter() - t0
         out[f"one_by_one_{mname}_gate_updates_per_s"] = gates / dt
     out["batched_vs_plain"] = (out["batched_gate_updates_per_s"]
                                / out["one_by_one_plain_launch_gate_updates_per_s"])
+    out["batched_vs_fused"] = (out["batched_gate_updates_per_s"]
+                               / out["one_by_one_fused_gate_updates_per_s"])
     return out
 
 
@@ -149,7 +166,7 @@ def main():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--n", type=int, default=14)
-    ap.add_argument("--methods", default="fused,plain_launch")
+    ap.add_argument("--methods", default="fused,fused_learned,plain_launch")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--calibrate-copy", action="store_true")
@@ -174,8 +191,18 @@ def main():
     for mname in args.methods.split(","):
         sim = dmsim_amd.Sim(n)
         sim.set_method(METHODS[mname])
+        if mname == "fused_learned":
+            import costdata
+            model = costdata.default_model()
+            if model is None:
+                log("[bench] fused_learned: no shipped cost model, skipped")
+                sim.close()
+                continue
+            sim.set_cost_model(model)
         sim.append_table(table)
         sim.upload()
+        if mname.startswith("fused"):
+            log(f"[bench] {mname}: plan {sim.plan_info()}")
         log(f"[bench] {mname}: n={n} gates={gates} warmup={args.warmup} steps={args.steps}")
         wall, dev_ms, stats, prof = timed_runs(sim, args.steps, args.warmup)
         ms_step = wall * 1e3 / args.steps

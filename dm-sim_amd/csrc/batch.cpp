@@ -175,24 +175,46 @@ int Batch::upload()
     pool_doubles_ = off;
     BTRY(hipMalloc(&pool_, pool_doubles_ * sizeof(double)));
     if (!blocks.empty()) BTRY(hipMalloc(&d_blocks_, blocks.size() * sizeof(FusedBlock)));
+    for (const Circ& c : circs_) tile_bits_[c.n] = default_tile_bits(c.n);
+    if (autotune_ && !blocks.empty()) {
+        BTRY(hipMemcpy(d_blocks_, blocks.data(), blocks.size() * sizeof(FusedBlock),
+                       hipMemcpyHostToDevice));
+        const int rc = autotune(blocks);
+        if (rc) return rc;
+    }
     BTRY(hipMalloc(&d_re_off_, re_off.size() * sizeof(uint64_t)));
     BTRY(hipMalloc(&d_energy_, circs_.size() * sizeof(double)));
 
     // launches: per block step, one per qubit count
     std::vector<BatchEntry> entries;
-    std::map<int, std::vector<int>> by_n;
-    for (int i = 0; i < (int)circs_.size(); i++) by_n[circs_[i].n].push_back(i);
-    for (size_t s = 0; s < max_steps; s++)
-        for (const auto& kv : by_n) {
-            Launch l{kv.first, 0, entries.size()};
+    // grouped by (qubit count, tile width): one launch per group and step
+    std::map<std::pair<int, int>, std::vector<int>> groups;
+    for (int i = 0; i < (int)circs_.size(); i++) groups[{circs_[i].n, 0}].push_back(i);
+    for (size_t s = 0; s < max_steps; s++) {
+        std::map<std::pair<int, int>, Launch> step;
+        for (const auto& kv : groups)
             for (int i : kv.second) {
                 const Circ& c = circs_[i];
                 if (s >= c.n_blocks) continue;
-                entries.push_back({pool_ + c.re_off, pool_ + c.im_off, d_blocks_ + c.blk_off + s});
-                l.count++;
+                const int m = (int)blocks[c.blk_off + s].m;
+                auto it = step.find({c.n, m});
+                if (it == step.end()) it = step.insert({{c.n, m}, Launch{c.n, m, 0, 0}}).first;
+                it->second.count++;
             }
-            if (l.count) launches_.push_back(l);
+        for (auto& kv : step) {
+            kv.second.entry_off = entries.size();
+            for (const auto& g : groups)
+                for (int i : g.second) {
+                    const Circ& c = circs_[i];
+                    if (s >= c.n_blocks || c.n != kv.first.first ||
+                        (int)blocks[c.blk_off + s].m != kv.first.second)
+                        continue;
+                    entries.push_back(
+                        {pool_ + c.re_off, pool_ + c.im_off, d_blocks_ + c.blk_off + s});
+                }
+            launches_.push_back(kv.second);
         }
+    }
     if (!entries.empty()) BTRY(hipMalloc(&d_entries_, entries.size() * sizeof(BatchEntry)));
 
     std::vector<EnergyJob> jobs;
@@ -234,11 +256,83 @@ int Batch::upload()
     return DMX_OK;
 }
 
+int Batch::set_autotune(bool on)
+{
+    if (uploaded_) {
+        set_error("set_autotune after upload");
+        return DMX_ESTATE;
+    }
+    autotune_ = on;
+    return DMX_OK;
+}
+
+int Batch::tile_bits(int n) const
+{
+    auto it = tile_bits_.find(n);
+    return it == tile_bits_.end() ? 0 : it->second;
+}
+
+// Times, for each qubit count, one batched launch of every circuit's first
+// block at each admissible super-tile width (5..min(8, n): every block of the
+// group must admit it, and m = 4 needs qubit 0 in the block), and gives all
+// blocks of that width class the fastest.  Runs before the pool is first
+// reset; the sweeps act on whatever the pool holds.
+int Batch::autotune(std::vector<FusedBlock>& blocks)
+{
+    std::map<int, std::vector<int>> by_n;
+    for (int i = 0; i < (int)circs_.size(); i++)
+        if (circs_[i].n_blocks > 0) by_n[circs_[i].n].push_back(i);
+    BTRY(hipMemsetAsync(pool_, 0, pool_doubles_ * sizeof(double), stream_));
+    size_t most = 0;
+    for (const auto& kv : by_n) most = std::max(most, kv.second.size());
+    BatchEntry* d_tmp = nullptr;
+    BTRY(hipMalloc(&d_tmp, most * sizeof(BatchEntry)));
+    int rc = DMX_OK;
+    for (const auto& kv : by_n) {
+        const int n = kv.first;
+        const int lo = n > 4 ? 5 : 4, hi = std::min(8, n);
+        if (lo >= hi) continue;
+        std::vector<BatchEntry> ents;
+        for (int i : kv.second)
+            ents.push_back({pool_ + circs_[i].re_off, pool_ + circs_[i].im_off,
+                             d_blocks_ + circs_[i].blk_off});
+        hipError_t e = hipMemcpy(d_tmp, ents.data(), ents.size() * sizeof(BatchEntry),
+                                 hipMemcpyHostToDevice);
+        float best_ms = 0.f;
+        int best_m = default_tile_bits(n);
+        for (int m = lo; m <= hi && e == hipSuccess; m++) {
+            e = launch_fused4_batch(d_tmp, (int)ents.size(), n, m, 4096, stream_);
+            if (e == hipSuccess) e = hipEventRecord(ev0_, stream_);
+            for (int r = 0; r < 3 && e == hipSuccess; r++)
+                e = launch_fused4_batch(d_tmp, (int)ents.size(), n, m, 4096, stream_);
+            if (e == hipSuccess) e = hipEventRecord(ev1_, stream_);
+            if (e == hipSuccess) e = hipEventSynchronize(ev1_);
+            float ms = 0.f;
+            if (e == hipSuccess) e = hipEventElapsedTime(&ms, ev0_, ev1_);
+            if (e == hipSuccess && (m == lo || ms < best_ms)) {
+                best_ms = ms;
+                best_m = m;
+            }
+        }
+        if (e != hipSuccess) {
+            set_error(std::string("batch autotune: ") + hipGetErrorString(e));
+            rc = DMX_EHIP;
+            break;
+        }
+        tile_bits_[n] = best_m;
+    }
+    (void)hipFree(d_tmp);
+    if (rc) return rc;
+    for (const Circ& c : circs_)
+        for (size_t b = 0; b < c.n_blocks; b++) blocks[c.blk_off + b].m = (uint32_t)tile_bits_[c.n];
+    return DMX_OK;
+}
+
 int Batch::enqueue()
 {
     BTRY(launch_pool_reset(pool_, pool_doubles_, d_re_off_, (int)circs_.size(), stream_));
     for (const Launch& l : launches_)
-        BTRY(launch_fused4_batch(d_entries_ + l.entry_off, l.count, l.n, 4096, stream_));
+        BTRY(launch_fused4_batch(d_entries_ + l.entry_off, l.count, l.n, l.m, 4096, stream_));
     BTRY(launch_pauli_energy(d_jobs_, d_terms_, (int)circs_.size(), d_energy_, stream_));
     return DMX_OK;
 }

@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <map>
 #include <string>
 #include <vector>
 
@@ -50,6 +51,12 @@ public:
     int energies(double* out) const;
     int get_dm(uint32_t id, double* re, double* im);
     void set_graph(bool on) { use_graph_ = on; }
+    // Tile-width autotuning at upload (default on): per qubit count, the
+    // super-tile width whose batched block launch times fastest on this GPU;
+    // off = the default width.
+    int set_autotune(bool on);
+    // Super-tile width used for circuits of n qubits (0 if none in the batch).
+    int tile_bits(int n) const;
 
     double last_ms() const { return last_ms_; }
     uint32_t launches() const { return (uint32_t)launches_.size() + 3; }
@@ -67,17 +74,21 @@ private:
     };
     struct Launch {
         int n;
+        int m;
         int count;
         size_t entry_off;
     };
     int enqueue();  // the whole run on stream_ (captured or eager)
+    int autotune(std::vector<FusedBlock>& blocks);
     void free_device();
 
     std::vector<Circ> circs_;
     std::vector<Launch> launches_;
     bool uploaded_ = false;
     bool use_graph_ = true;
+    bool autotune_ = true;
     bool ran_ = false;
+    std::map<int, int> tile_bits_;  // qubit count -> super-tile width
 
     double* pool_ = nullptr;
     uint64_t pool_doubles_ = 0;

@@ -8,10 +8,19 @@
 #include "dmsim_amd.h"
 #include "batch.hpp"
 #include "engine.hpp"
+#include "fused.hpp"
+#include "mlp.hpp"
 
 struct dmx_sim {
     dmx::Engine eng;
 };
+
+struct dmx_cost {
+    dmx::CostModel m;
+};
+
+static_assert(DMX_COST_FEATURES == dmx::kMlpFeatures, "feature count");
+static_assert(DMX_COST_PARAMS == dmx::kMlpParams, "parameter count");
 
 struct dmx_batch {
     dmx::Batch b;
@@ -253,6 +262,154 @@ int dmx_profile_get(dmx_sim* sim, uint32_t cls, char* name, size_t cap, uint32_t
     return DMX_OK;
 }
 
+int dmx_cost_create(uint32_t seed, dmx_cost** out)
+{
+    if (!out) {
+        dmx::set_error("null output pointer");
+        return DMX_EINVAL;
+    }
+    *out = nullptr;
+    return guarded([&] {
+        dmx_cost* c = new dmx_cost();
+        int rc = c->m.init(seed);
+        if (rc) {
+            delete c;
+            return rc;
+        }
+        *out = c;
+        return DMX_OK;
+    });
+}
+
+int dmx_cost_destroy(dmx_cost* model)
+{
+    delete model;
+    return DMX_OK;
+}
+
+int dmx_cost_features(uint32_t n_qubits, int32_t kind, int32_t m, uint32_t qmask, int32_t target,
+                      int32_t control, int32_t slow, float* out)
+{
+    if (!out || n_qubits < 1 || n_qubits > 16 || kind < 0 || kind > DMX_COST_QUAD) {
+        dmx::set_error("cost features need an output, 1 <= n <= 16 and a known kind");
+        return DMX_EINVAL;
+    }
+    const int n = (int)n_qubits;
+    if (kind == DMX_COST_FUSED) {
+        if (n < 4 || __builtin_popcount(qmask) != 4 || (qmask >> n) != 0 ||
+            !dmx::tile_bits_ok(n, m, qmask)) {
+            dmx::set_error("fused features need n >= 4, four qubits < n and 4 <= m <= min(8, n) "
+                           "(m = 4 only with qubit 0 in the block)");
+            return DMX_EINVAL;
+        }
+    } else {
+        const bool two = kind == DMX_COST_CX || kind == DMX_COST_QUAD;
+        if (target < 0 || target >= n ||
+            (two && (control < 0 || control >= n || control == target))) {
+            dmx::set_error("sweep features need a target (and a distinct control) below n");
+            return DMX_EINVAL;
+        }
+    }
+    dmx::cost_features(n, kind, m, qmask, target, control, slow, out);
+    return DMX_OK;
+}
+
+int dmx_cost_train(dmx_cost* model, uint32_t count, const float* X, const float* y,
+                   uint32_t epochs, uint32_t batch, float lr, uint32_t seed, float* final_loss)
+{
+    if (!model || !X || !y) {
+        dmx::set_error("null model or data");
+        return DMX_EINVAL;
+    }
+    if (count > (1u << 24) || epochs > 100000 || batch > (1u << 24)) {
+        dmx::set_error("training sizes out of range");
+        return DMX_EINVAL;
+    }
+    return guarded([&] {
+        return model->m.train((int)count, X, y, (int)epochs, (int)batch, lr, seed, final_loss);
+    });
+}
+
+int dmx_cost_predict(dmx_cost* model, uint32_t count, const float* X, float* out)
+{
+    if (!model || (count && (!X || !out)) || count > (1u << 24)) {
+        dmx::set_error("null model or data, or too many samples");
+        return DMX_EINVAL;
+    }
+    return guarded([&] { return model->m.predict((int)count, X, out); });
+}
+
+int dmx_cost_gradient(dmx_cost* model, uint32_t count, const float* X, const float* y,
+                      float* out)
+{
+    if (!model || !X || !y || !out || count > (1u << 24)) {
+        dmx::set_error("null model or data, or too many samples");
+        return DMX_EINVAL;
+    }
+    return guarded([&] { return model->m.gradient((int)count, X, y, out); });
+}
+
+int dmx_cost_get_params(const dmx_cost* model, float* params, float* mean, float* dev)
+{
+    if (!model) {
+        dmx::set_error("null model");
+        return DMX_EINVAL;
+    }
+    if (params) model->m.get_params(params);
+    float mu = 0.f, sigma = 1.f;
+    model->m.target_scale(&mu, &sigma);
+    if (mean) *mean = mu;
+    if (dev) *dev = sigma;
+    return DMX_OK;
+}
+
+int dmx_cost_set_params(dmx_cost* model, const float* params, float mean, float dev)
+{
+    if (!model || !params || !(dev > 0.f)) {
+        dmx::set_error("null model or parameters, or deviation <= 0");
+        return DMX_EINVAL;
+    }
+    model->m.set_target_scale(mean, dev);
+    return guarded([&] { return model->m.set_params(params); });
+}
+
+int dmx_set_cost_model(dmx_sim* sim, dmx_cost* model)
+{
+    if (!sim) return null_handle();
+    sim->eng.set_cost_model(model ? &model->m : nullptr);
+    return DMX_OK;
+}
+
+int dmx_set_tile_bits(dmx_sim* sim, uint32_t m)
+{
+    if (!sim) return null_handle();
+    return sim->eng.set_tile_bits((int)m);
+}
+
+int dmx_plan_info(const dmx_sim* sim, uint32_t* fused_blocks, uint32_t* plain_blocks,
+                  uint32_t* wide_gates, double* predicted_us, double* greedy_us, uint32_t* tiles)
+{
+    if (!sim) return null_handle();
+    const dmx::PlanInfo& pi = sim->eng.plan_info();
+    if (fused_blocks) *fused_blocks = pi.fused_blocks;
+    if (plain_blocks) *plain_blocks = pi.plain_blocks;
+    if (wide_gates) *wide_gates = pi.wide_gates;
+    if (predicted_us) *predicted_us = pi.predicted_us;
+    if (greedy_us) *greedy_us = pi.greedy_us;
+    if (tiles)
+        for (int m = 4; m <= 8; m++) tiles[m - 4] = pi.tile_hist[m];
+    return DMX_OK;
+}
+
+int dmx_time_launch(dmx_sim* sim, int32_t kind, int32_t m, uint32_t qmask, int32_t target,
+                    int32_t control, int32_t slow, uint32_t reps, double* us)
+{
+    if (!sim) return null_handle();
+    return guarded([&] {
+        return sim->eng.time_launch(kind, m, qmask, target, control, slow, (int)reps, us);
+    });
+}
+
 int dmx_batch_create(dmx_batch** out)
 {
     if (!out) {
@@ -315,6 +472,22 @@ int dmx_batch_set_graph(dmx_batch* b, int on)
 {
     if (!b) return null_handle();
     b->b.set_graph(on != 0);
+    return DMX_OK;
+}
+
+int dmx_batch_set_autotune(dmx_batch* b, int on)
+{
+    if (!b) return null_handle();
+    return b->b.set_autotune(on != 0);
+}
+
+int dmx_batch_tile_bits(const dmx_batch* b, uint32_t n_qubits, uint32_t* m)
+{
+    if (!b || !m) {
+        dmx::set_error("null batch handle or output");
+        return DMX_EINVAL;
+    }
+    *m = (uint32_t)b->b.tile_bits((int)n_qubits);
     return DMX_OK;
 }
 

@@ -11,6 +11,8 @@
 
 #include "dmsim_amd.h"
 #include "fuse.hpp"
+#include "fused.hpp"
+#include "mlp.hpp"
 #include "sweep.hpp"
 
 namespace dmx {
@@ -38,6 +40,8 @@ Engine::~Engine()
     if (d_im_) (void)hipFree(d_im_);
     if (d_diag_) (void)hipFree(d_diag_);
     if (d_ops_) (void)hipFree(d_ops_);
+    if (d_probe_blk_) (void)hipFree(d_probe_blk_);
+    if (d_probe_op_) (void)hipFree(d_probe_op_);
     if (ev0_) (void)hipEventDestroy(ev0_);
     if (ev1_) (void)hipEventDestroy(ev1_);
     if (stream_) (void)hipStreamDestroy(stream_);
@@ -127,6 +131,7 @@ int Engine::upload()
         if (!lower_gate(g, prims_)) wide = true;
     delete plan_;
     plan_ = nullptr;
+    plan_info_ = PlanInfo{};
     if (wide && method_ != DMX_METHOD_FUSED) {
         // a k >= 3 matrix has no primitive sweep; it needs the fused executor
         set_error("matrix gates on 3..5 qubits need DMX_METHOD_FUSED");
@@ -140,7 +145,8 @@ int Engine::upload()
     }
     if (method_ == DMX_METHOD_FUSED) {
         // no plan (SRN in the circuit, or n < 4): the run uses the plain sweeps
-        int rc = build_fused_plan(circuit_, n_, fusion_max_k_, &plan_);
+        int rc = build_fused_plan(circuit_, n_, fusion_max_k_, model_, tile_bits_, &plan_,
+                                  &plan_info_);
         if (rc) return rc;
         if (plan_) {
             rc = plan_->upload(stream_);
@@ -333,6 +339,16 @@ int Engine::set_method(int m)
     return DMX_OK;
 }
 
+int Engine::set_tile_bits(int m)
+{
+    if (m != 0 && (m < 4 || m > 8)) {
+        set_error("tile width must be 0 (automatic) or 4..8");
+        return DMX_EINVAL;
+    }
+    tile_bits_ = m;
+    return DMX_OK;
+}
+
 int Engine::set_fusion_max_k(uint32_t k)
 {
     if (k < 1 || k > 5) {
@@ -340,6 +356,88 @@ int Engine::set_fusion_max_k(uint32_t k)
         return DMX_EINVAL;
     }
     fusion_max_k_ = (int)k;
+    return DMX_OK;
+}
+
+int Engine::time_launch(int kind, int m, uint32_t qmask, int target, int control, int slow,
+                        int reps, double* us)
+{
+    if (reps < 1 || reps > 100000 || !us) {
+        set_error("time_launch needs 1 <= reps <= 100000 and an output");
+        return DMX_EINVAL;
+    }
+    const double pi = 3.14159265358979323846;
+    PrimOp op;
+    std::memset(&op, 0, sizeof(op));
+    op.q1 = -1;
+    if (kind == CK_FUSED) {
+        if (n_ < 4 || __builtin_popcount(qmask) != 4 || (qmask >> n_) != 0 ||
+            !tile_bits_ok(n_, m, qmask)) {
+            set_error("fused timing needs n >= 4, four qubits < n and 4 <= m <= min(8, n) "
+                      "(m = 4 only with qubit 0 in the block)");
+            return DMX_EINVAL;
+        }
+    } else if (kind >= CK_DENSE && kind <= CK_QUAD) {
+        const bool two = kind == CK_CX || kind == CK_QUAD;
+        if (target < 0 || target >= n_ || (two && (control < 0 || control >= n_ || control == target))) {
+            set_error("sweep timing needs a target (and a distinct control) below n");
+            return DMX_EINVAL;
+        }
+        op.q0 = target;
+        if (kind == CK_DENSE) {  // Hadamard-like 2x2
+            op.kind = PK_C1;
+            const double h = 1.0 / std::sqrt(2.0);
+            op.c[0] = h, op.c[2] = h, op.c[4] = h, op.c[6] = -h;
+        } else if (kind == CK_DIAG) {
+            op.kind = PK_U1;
+            op.c[0] = std::cos(0.3);
+            op.c[1] = std::sin(0.3);
+        } else if (kind == CK_CX) {
+            op.kind = PK_X;
+            op.q1 = control;
+        } else {  // 4x4 DFT / 2
+            op.kind = PK_C2;
+            op.q1 = control;
+            for (int r = 0; r < 4; r++)
+                for (int c = 0; c < 4; c++) {
+                    op.c[2 * (4 * r + c)] = 0.5 * std::cos(2 * pi * r * c / 4);
+                    op.c[2 * (4 * r + c) + 1] = 0.5 * std::sin(2 * pi * r * c / 4);
+                }
+        }
+    } else {
+        set_error("unknown launch kind " + std::to_string(kind));
+        return DMX_EINVAL;
+    }
+    if (!d_probe_blk_) HIPTRY(hipMalloc(&d_probe_blk_, sizeof(FusedBlock)));
+    if (!d_probe_op_) HIPTRY(hipMalloc(&d_probe_op_, sizeof(PrimOp)));
+    if (kind == CK_FUSED) {  // 16 x 16 DFT / 4
+        FusedBlock fb;
+        std::memset(&fb, 0, sizeof(fb));
+        fb.qmask = qmask;
+        fb.m = (uint32_t)m;
+        for (int r = 0; r < 16; r++)
+            for (int c = 0; c < 16; c++) {
+                fb.ur[16 * r + c] = 0.25 * std::cos(2 * pi * r * c / 16);
+                fb.ui[16 * r + c] = 0.25 * std::sin(2 * pi * r * c / 16);
+            }
+        HIPTRY(hipMemcpy(d_probe_blk_, &fb, sizeof(fb), hipMemcpyHostToDevice));
+    } else {
+        HIPTRY(hipMemcpy(d_probe_op_, &op, sizeof(op), hipMemcpyHostToDevice));
+    }
+    auto launch = [&]() {
+        return kind == CK_FUSED
+                   ? launch_fused4(d_re_, d_im_, d_probe_blk_, n_, m, kFusedGridCap, stream_)
+                   : launch_plain_one(d_re_, d_im_, d_probe_op_, op, n_, slow != 0, kSweepGridCap,
+                                      stream_);
+    };
+    HIPTRY(launch());
+    HIPTRY(hipEventRecord(ev0_, stream_));
+    for (int r = 0; r < reps; r++) HIPTRY(launch());
+    HIPTRY(hipEventRecord(ev1_, stream_));
+    HIPTRY(hipEventSynchronize(ev1_));
+    float ms = 0.f;
+    HIPTRY(hipEventElapsedTime(&ms, ev0_, ev1_));
+    *us = 1000.0 * ms / reps;
     return DMX_OK;
 }
 

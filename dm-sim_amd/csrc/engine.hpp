@@ -6,13 +6,12 @@
 #include <string>
 #include <vector>
 
+#include "fuse.hpp"
 #include "lower.hpp"
 #include "prim.hpp"
 #include "prof.hpp"
 
 namespace dmx {
-
-struct FusedPlan;  // fuse.hpp
 
 class Engine {
 public:
@@ -33,6 +32,21 @@ public:
     int expect_z(double* out);
     int set_method(int m);
     int set_fusion_max_k(uint32_t k);
+    // Cost model for the fused planner (not owned; nullptr = greedy plan).
+    // Takes effect at the next upload().
+    void set_cost_model(CostModel* model) { model_ = model; }
+    // Fixed super-tile width for every fused block that admits it (0 = chosen
+    // by the model or the default).  Takes effect at the next upload().
+    int set_tile_bits(int m);
+    const PlanInfo& plan_info() const { return plan_info_; }
+    // Timing-data generator: average time (microseconds) of `reps` back-to-back
+    // launches of one kernel configuration on this simulation's rho, after one
+    // untimed launch.  kind is a CostKind; fused: m, qmask (four qubits);
+    // sweeps: target, control (cx control or 4x4 low qubit, else -1) and slow
+    // (index side).  rho is left transformed by unitaries on one or both
+    // sides: reset it before simulating.
+    int time_launch(int kind, int m, uint32_t qmask, int target, int control, int slow, int reps,
+                    double* us);
     std::string dump() const;
 
     uint32_t n_qubits() const { return (uint32_t)n_; }
@@ -70,6 +84,11 @@ private:
     std::vector<Gate> circuit_;
     std::vector<PrimOp> prims_;
     FusedPlan* plan_ = nullptr;
+    CostModel* model_ = nullptr;
+    int tile_bits_ = 0;
+    PlanInfo plan_info_;
+    FusedBlock* d_probe_blk_ = nullptr;  // time_launch's operator records
+    PrimOp* d_probe_op_ = nullptr;
     LaunchProfiler prof_;
 
     double last_ms_ = 0.0;

@@ -8,6 +8,12 @@
 // qubits it does not use (identity there) and its 16 x 16 matrix is the
 // product of the member gates' matrices, each composed from the gate's own
 // primitive sweeps (so C4X, C3SQRTX, R keep the reference's semantics).
+//
+// Execution choice: without a cost model every block runs as one fused sweep
+// at the default tile width.  With one, every block is priced (one batched
+// inference over all candidates) as a fused sweep at each tile width and as
+// its member gates' primitive sweeps on both indices, and the cheapest
+// prediction is taken.
 #include "fuse.hpp"
 
 #include <algorithm>
@@ -17,6 +23,7 @@
 #include "dmsim_amd.h"
 #include "engine.hpp"
 #include "fused.hpp"
+#include "mlp.hpp"
 #include "sweep.hpp"
 
 namespace dmx {
@@ -26,9 +33,9 @@ namespace {
 using cd = std::complex<double>;
 
 // A run is a sequence of steps: a fused block (one sweep), or a plain segment
-// of primitive sweeps for a gate wider than the block width, applied to the
-// fast index and then to the slow index (valid for complex-linear gates, whose
-// row and column actions commute).
+// of primitive sweeps (a gate wider than the block width, or a block the cost
+// model runs unfused), applied to the fast index and then to the slow index
+// (valid for complex-linear gates, whose row and column actions commute).
 struct Step {
     bool fused;
     int index;  // block index, or first primitive of the segment
@@ -43,7 +50,7 @@ struct Plan : FusedPlan {
     std::vector<Step> steps;
     FusedBlock* d_blocks = nullptr;
     PrimOp* d_prims = nullptr;
-    int max_blocks = 4096;
+    int max_blocks = kFusedGridCap;
 
     ~Plan() override
     {
@@ -84,15 +91,16 @@ struct Plan : FusedPlan {
             hipError_t e = hipSuccess;
             if (st.fused) {
                 prof.begin(KC_FUSED4, sweep_bytes);
-                e = launch_fused4(re, im, d_blocks + st.index, nq, max_blocks, stream);
+                e = launch_fused4(re, im, d_blocks + st.index, nq, (int)blocks[st.index].m,
+                                  max_blocks, stream);
                 prof.end();
                 la++;
             } else {
                 for (int side = 0; side < 2 && e == hipSuccess; side++)
                     for (int t = st.index; t < st.index + st.count && e == hipSuccess; t++) {
                         prof.begin(prim_class(prims[t]), prim_bytes(prims[t], nq));
-                        e = launch_plain_one(re, im, d_prims + t, prims[t], nq, side == 1, 8192,
-                                             stream);
+                        e = launch_plain_one(re, im, d_prims + t, prims[t], nq, side == 1,
+                                             kSweepGridCap, stream);
                         prof.end();
                         la++;
                     }
@@ -222,37 +230,145 @@ int plan_blocks(const std::vector<Gate>& circuit, int n, int max_k, BlockPlan& o
         FusedBlock fb;
         std::memset(&fb, 0, sizeof(fb));
         fb.qmask = qmask & all;
+        fb.m = (uint32_t)default_tile_bits(n);
         for (int i = 0; i < 256; i++) {
             fb.ur[i] = U[i].real();
             fb.ui[i] = U[i].imag();
         }
         out.blocks.push_back(fb);
         out.gates_per_block.push_back((uint32_t)pb.gates.size());
+        out.members.push_back(pb.gates);
     }
     return DMX_OK;
 }
 
-int build_fused_plan(const std::vector<Gate>& circuit, int n, int max_k, FusedPlan** out)
+namespace {
+
+int cost_kind(const PrimOp& p)
+{
+    switch (prim_class(p)) {
+    case KC_QUAD: return CK_QUAD;
+    case KC_CX: return CK_CX;
+    case KC_PAIR_POS1: return CK_DIAG;
+    default: return CK_DENSE;
+    }
+}
+
+// Learned choice per block: fused at tile width m (4..min(8, n)) or its
+// member gates' primitive sweeps on both indices.  Sets blocks[b].m, fills
+// plain[b] with the primitives of blocks that run unfused.
+// A candidate replaces the default only if predicted at least this much faster
+// (the model's median error on held-out launches is about 10 %).
+constexpr double kMargin = 0.92;
+
+int choose_executions(const std::vector<Gate>& circuit, int n, BlockPlan& bp, CostModel& model,
+                      std::vector<std::vector<PrimOp>>& plain, PlanInfo& info)
+{
+    const int nb = (int)bp.blocks.size();
+    const int m_hi = std::min(8, n);
+    const int n_m = m_hi - 4 + 1;
+    std::vector<std::vector<float>> feats;
+    std::vector<size_t> first(nb);
+    std::vector<int> n_plain(nb, -1);  // feature rows of the unfused option, -1 if none
+    plain.assign(nb, {});
+    float f[kMlpFeatures];
+    for (int b = 0; b < nb; b++) {
+        first[b] = feats.size();
+        for (int m = 4; m <= m_hi; m++) {
+            // an inadmissible width is priced like a valid one and never chosen
+            const int mm = tile_bits_ok(n, m, bp.blocks[b].qmask) ? m : m_hi;
+            cost_features(n, CK_FUSED, mm, bp.blocks[b].qmask, 0, -1, 0, f);
+            feats.emplace_back(f, f + kMlpFeatures);
+        }
+        std::vector<PrimOp> ps;
+        bool ok = true;
+        for (int gi : bp.members[b]) ok = ok && lower_gate(circuit[gi], ps);
+        if (!ok) continue;
+        for (int side = 0; side < 2; side++)
+            for (const PrimOp& p : ps) {
+                cost_features(n, cost_kind(p), 0, 0, p.q0, p.q1, side, f);
+                feats.emplace_back(f, f + kMlpFeatures);
+            }
+        n_plain[b] = 2 * (int)ps.size();
+        plain[b] = std::move(ps);
+    }
+    std::vector<float> us;
+    int rc = model.price(feats, us);
+    if (rc) return rc;
+    const int m_default = default_tile_bits(n);
+    for (int b = 0; b < nb; b++) {
+        const float* t = us.data() + first[b];
+        // leave the default only for a predicted gain above the model's noise
+        int best_m = m_default;
+        double best = t[m_default - 4];
+        for (int i = 0; i < n_m; i++)
+            if (tile_bits_ok(n, 4 + i, bp.blocks[b].qmask) && t[i] < kMargin * t[m_default - 4] &&
+                t[i] < best) {
+                best_m = 4 + i;
+                best = t[i];
+            }
+        info.greedy_us += t[m_default - 4];
+        bool use_plain = false;
+        if (n_plain[b] >= 0) {
+            double s = 0;
+            for (int i = 0; i < n_plain[b]; i++) s += t[n_m + i];
+            if (s < kMargin * best) {
+                best = s;
+                use_plain = true;
+            }
+        }
+        info.predicted_us += best;
+        if (use_plain) {
+            info.plain_blocks++;
+        } else {
+            bp.blocks[b].m = (uint32_t)best_m;
+            plain[b].clear();
+        }
+    }
+    return DMX_OK;
+}
+
+}  // namespace
+
+int build_fused_plan(const std::vector<Gate>& circuit, int n, int max_k, CostModel* model,
+                     int tile_bits, FusedPlan** out, PlanInfo* info)
 {
     *out = nullptr;
+    PlanInfo pi;
     BlockPlan bp;
     bool fusable = false;
     int rc = plan_blocks(circuit, n, max_k, bp, fusable);
     if (rc || !fusable) return rc;
+    std::vector<std::vector<PrimOp>> plain(bp.blocks.size());
+    if (tile_bits > 0) {
+        for (FusedBlock& fb : bp.blocks)
+            if (tile_bits_ok(n, tile_bits, fb.qmask)) fb.m = (uint32_t)tile_bits;
+    } else if (model) {
+        rc = choose_executions(circuit, n, bp, *model, plain, pi);
+        if (rc) return rc;
+    }
     Plan* plan = new Plan();
     plan->n = n;
     plan->blocks = std::move(bp.blocks);
     plan->gates_per_block = std::move(bp.gates_per_block);
     for (const auto& o : bp.order) {
-        if (o.first == 0) {
+        const int first = (int)plan->prims.size();
+        if (o.first == 0 && plain[o.second].empty()) {
             plan->steps.push_back({true, o.second, 0});
-        } else {
-            const int first = (int)plan->prims.size();
-            lower_gate(circuit[o.second], plan->prims);
-            plan->steps.push_back({false, first, (int)plan->prims.size() - first});
+            pi.fused_blocks++;
+            pi.tile_hist[plan->blocks[o.second].m]++;
+            continue;
         }
+        if (o.first == 0) {
+            plan->prims.insert(plan->prims.end(), plain[o.second].begin(), plain[o.second].end());
+        } else {
+            lower_gate(circuit[o.second], plan->prims);
+            pi.wide_gates++;
+        }
+        plan->steps.push_back({false, first, (int)plan->prims.size() - first});
     }
     *out = plan;
+    if (info) *info = pi;
     return DMX_OK;
 }
 

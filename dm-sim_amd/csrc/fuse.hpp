@@ -26,6 +26,7 @@ struct FusedPlan {
 struct BlockPlan {
     std::vector<FusedBlock> blocks;
     std::vector<uint32_t> gates_per_block;
+    std::vector<std::vector<int>> members;  // circuit indices of each block's gates, in order
     std::vector<std::pair<int, int>> order;
 };
 
@@ -33,6 +34,27 @@ struct BlockPlan {
 // (and no blocks) when the circuit must run the plain method (SRN, n < 4).
 int plan_blocks(const std::vector<Gate>& circuit, int n, int max_k, BlockPlan& out, bool& fusable);
 
-int build_fused_plan(const std::vector<Gate>& circuit, int n, int max_k, FusedPlan** out);
+class CostModel;  // mlp.hpp
+
+// What the planner chose.  predicted_us is the cost model's estimate of the
+// run's launches (0 without a model); greedy_us its estimate for every block
+// fused at the default tile width.
+struct PlanInfo {
+    uint32_t fused_blocks = 0;
+    uint32_t plain_blocks = 0;   // blocks run as primitive sweeps on both indices
+    uint32_t wide_gates = 0;     // gates wider than a block (primitive sweeps)
+    uint32_t tile_hist[9] = {};  // fused blocks per tile width m (4..8)
+    double predicted_us = 0.0;
+    double greedy_us = 0.0;
+};
+
+// Plan of a circuit for the fused executor.  With a cost model, each block is
+// priced as a fused launch at every tile width and as its primitive sweeps,
+// and the cheapest is taken; without one, every block is fused at the
+// default tile width (the greedy plan).
+// tile_bits > 0 fixes every fused block's tile width where that width is
+// admissible for the block (tile_bits_ok), and takes precedence over the model.
+int build_fused_plan(const std::vector<Gate>& circuit, int n, int max_k, CostModel* model,
+                     int tile_bits, FusedPlan** out, PlanInfo* info);
 
 }  // namespace dmx

@@ -4,9 +4,11 @@
 //
 // Tile: the 16 x 16 entries whose row and column indices range over the
 // block's four qubits Q (other bits fixed).  A workgroup stages a super-tile
-// in LDS: those 16 rows x 256 columns, the columns spanning Q plus the lowest
-// free column bits, so every global access is a contiguous run of at least 16
-// doubles (the four lowest column bits are always inside the super-tile).
+// in LDS: those 16 rows x 2^m columns, the columns spanning Q plus the lowest
+// m - 4 free column bits.  At m = 8 the four lowest column bits are always
+// inside the super-tile, so every global access is a contiguous run of at
+// least 16 doubles.  Threads move column pairs, so column bit 0 must be in
+// the super-tile (m > 4, or qubit 0 in Q).
 // Each of the 16 tiles of a super-tile is transformed by one wave:
 //   X  = T U^T      (12 MFMAs: complex product from three real products)
 //   T' = conj(U) X  (12 MFMAs)
@@ -225,21 +227,17 @@ __global__ void __launch_bounds__(256, 2) k_fused4_batch(const BatchEntry* __res
 
 namespace {
 
-uint64_t supertiles(int n)
-{
-    const int m = n < 8 ? n : 8;
-    return (1ull << (n - 4)) * (1ull << (n - m));
-}
+uint64_t supertiles(int n, int m) { return (1ull << (n - 4)) * (1ull << (n - m)); }
 
 }  // namespace
 
-hipError_t launch_fused4(double* re, double* im, const FusedBlock* d_blk, int n, int max_blocks,
-                         hipStream_t stream)
+hipError_t launch_fused4(double* re, double* im, const FusedBlock* d_blk, int n, int m,
+                         int max_blocks, hipStream_t stream)
 {
-    if (n < 4) return hipErrorInvalidValue;
-    const uint64_t n_st = supertiles(n);
+    if (n < 4 || m < 4 || m > 8 || m > n) return hipErrorInvalidValue;
+    const uint64_t n_st = supertiles(n, m);
     const unsigned blocks = (unsigned)(n_st < (uint64_t)max_blocks ? n_st : (uint64_t)max_blocks);
-    switch (n < 8 ? n : 8) {
+    switch (m) {
     case 4: k_fused4<4><<<blocks, 256, 0, stream>>>(re, im, d_blk, n); break;
     case 5: k_fused4<5><<<blocks, 256, 0, stream>>>(re, im, d_blk, n); break;
     case 6: k_fused4<6><<<blocks, 256, 0, stream>>>(re, im, d_blk, n); break;
@@ -249,14 +247,15 @@ hipError_t launch_fused4(double* re, double* im, const FusedBlock* d_blk, int n,
     return hipGetLastError();
 }
 
-hipError_t launch_fused4_batch(const BatchEntry* d_entries, int count, int n, int max_blocks,
-                               hipStream_t stream)
+hipError_t launch_fused4_batch(const BatchEntry* d_entries, int count, int n, int m,
+                               int max_blocks, hipStream_t stream)
 {
-    if (n < 4 || count < 1 || count > 65535) return hipErrorInvalidValue;
-    const uint64_t n_st = supertiles(n);
+    if (n < 4 || m < 4 || m > 8 || m > n || count < 1 || count > 65535)
+        return hipErrorInvalidValue;
+    const uint64_t n_st = supertiles(n, m);
     const dim3 grid((unsigned)(n_st < (uint64_t)max_blocks ? n_st : (uint64_t)max_blocks),
                     (unsigned)count);
-    switch (n < 8 ? n : 8) {
+    switch (m) {
     case 4: k_fused4_batch<4><<<grid, 256, 0, stream>>>(d_entries, n); break;
     case 5: k_fused4_batch<5><<<grid, 256, 0, stream>>>(d_entries, n); break;
     case 6: k_fused4_batch<6><<<grid, 256, 0, stream>>>(d_entries, n); break;

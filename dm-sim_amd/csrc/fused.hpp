@@ -10,7 +10,8 @@ namespace dmx {
 // lowest qubit of qmask).
 struct FusedBlock {
     uint32_t qmask;  // exactly four qubits
-    uint32_t pad[3];
+    uint32_t m;      // super-tile column bits (the tile schedule), see tile_bits_ok
+    uint32_t pad[2];
     double ur[256];
     double ui[256];
 };
@@ -22,12 +23,24 @@ struct BatchEntry {
     const FusedBlock* blk;
 };
 
-// rho <- U rho U^dagger for one block (n >= 4).
-hipError_t launch_fused4(double* re, double* im, const FusedBlock* d_blk, int n, int max_blocks,
-                         hipStream_t stream);
+// Default super-tile width for n qubits.
+inline int default_tile_bits(int n) { return n < 8 ? n : 8; }
+
+// Super-tile column bits are the block's four qubits plus the lowest other
+// qubits, m in all.  Global accesses move column pairs (bit 0 must be a
+// column bit), so m = 4 needs qubit 0 in the block.
+inline bool tile_bits_ok(int n, int m, uint32_t qmask)
+{
+    return m >= 4 && m <= (n < 8 ? n : 8) && (m > 4 || (qmask & 1u));
+}
+
+// rho <- U rho U^dagger for one block (n >= 4), staged in super-tiles of
+// 16 rows x 2^m columns.
+hipError_t launch_fused4(double* re, double* im, const FusedBlock* d_blk, int n, int m,
+                         int max_blocks, hipStream_t stream);
 
 // The same for `count` circuits of n qubits each (one launch).
-hipError_t launch_fused4_batch(const BatchEntry* d_entries, int count, int n, int max_blocks,
-                               hipStream_t stream);
+hipError_t launch_fused4_batch(const BatchEntry* d_entries, int count, int n, int m,
+                               int max_blocks, hipStream_t stream);
 
 }  // namespace dmx

@@ -17,6 +17,8 @@ LIB_PATH = os.path.join(HERE, "libdmsim_amd.so")
 
 DMX_OK, DMX_EINVAL, DMX_ESTATE, DMX_EHIP, DMX_ENOMEM = 0, -1, -2, -3, -4
 METHOD_PLAIN, METHOD_PLAIN_LAUNCH, METHOD_FUSED = 0, 1, 2
+COST_FUSED, COST_DENSE, COST_DIAG, COST_CX, COST_QUAD = 0, 1, 2, 3, 4
+COST_FEATURES, COST_PARAMS = 12, 5057
 
 # every symbol include/dmsim_amd.h declares
 EXPORTS = [
@@ -24,9 +26,13 @@ EXPORTS = [
     "dmx_clear_circuit", "dmx_reset", "dmx_reset_dm", "dmx_measure", "dmx_get_dm",
     "dmx_get_diag", "dmx_expect_z", "dmx_info", "dmx_dump", "dmx_set_method",
     "dmx_set_fusion_max_k", "dmx_last_run_stats", "dmx_set_profiling", "dmx_profile_get",
+    "dmx_cost_create", "dmx_cost_destroy", "dmx_cost_features", "dmx_cost_train",
+    "dmx_cost_predict", "dmx_cost_gradient", "dmx_cost_get_params", "dmx_cost_set_params",
+    "dmx_set_cost_model", "dmx_set_tile_bits", "dmx_plan_info", "dmx_time_launch",
     "dmx_batch_create", "dmx_batch_destroy", "dmx_batch_add_circuit", "dmx_batch_add_pauli",
-    "dmx_batch_upload", "dmx_batch_set_graph", "dmx_batch_run", "dmx_batch_energies",
-    "dmx_batch_get_dm", "dmx_batch_stats", "dmx_last_error",
+    "dmx_batch_upload", "dmx_batch_set_graph", "dmx_batch_set_autotune", "dmx_batch_tile_bits",
+    "dmx_batch_run", "dmx_batch_energies", "dmx_batch_get_dm", "dmx_batch_stats",
+    "dmx_last_error",
 ]
 
 _lib = None
@@ -67,6 +73,23 @@ def load():
     lib.dmx_set_profiling.argtypes = [vp, C.c_int]
     lib.dmx_profile_get.argtypes = [vp, C.c_uint32, C.c_char_p, C.c_size_t, P(C.c_uint32),
                                     P(C.c_double), P(C.c_double), P(C.c_uint32)]
+    f32p = P(C.c_float)
+    lib.dmx_cost_create.argtypes = [C.c_uint32, P(vp)]
+    lib.dmx_cost_destroy.argtypes = [vp]
+    lib.dmx_cost_features.argtypes = [C.c_uint32, C.c_int32, C.c_int32, C.c_uint32, C.c_int32,
+                                      C.c_int32, C.c_int32, f32p]
+    lib.dmx_cost_train.argtypes = [vp, C.c_uint32, f32p, f32p, C.c_uint32, C.c_uint32,
+                                   C.c_float, C.c_uint32, f32p]
+    lib.dmx_cost_predict.argtypes = [vp, C.c_uint32, f32p, f32p]
+    lib.dmx_cost_gradient.argtypes = [vp, C.c_uint32, f32p, f32p, f32p]
+    lib.dmx_cost_get_params.argtypes = [vp, f32p, f32p, f32p]
+    lib.dmx_cost_set_params.argtypes = [vp, f32p, C.c_float, C.c_float]
+    lib.dmx_set_cost_model.argtypes = [vp, vp]
+    lib.dmx_set_tile_bits.argtypes = [vp, C.c_uint32]
+    lib.dmx_plan_info.argtypes = [vp, P(C.c_uint32), P(C.c_uint32), P(C.c_uint32),
+                                  P(C.c_double), P(C.c_double), P(C.c_uint32)]
+    lib.dmx_time_launch.argtypes = [vp, C.c_int32, C.c_int32, C.c_uint32, C.c_int32, C.c_int32,
+                                    C.c_int32, C.c_uint32, P(C.c_double)]
     lib.dmx_batch_create.argtypes = [P(vp)]
     lib.dmx_batch_destroy.argtypes = [vp]
     lib.dmx_batch_add_circuit.argtypes = [vp, C.c_uint32, C.c_uint32, P(C.c_int32),
@@ -75,6 +98,8 @@ def load():
     for name in ("dmx_batch_upload", "dmx_batch_run"):
         getattr(lib, name).argtypes = [vp]
     lib.dmx_batch_set_graph.argtypes = [vp, C.c_int]
+    lib.dmx_batch_set_autotune.argtypes = [vp, C.c_int]
+    lib.dmx_batch_tile_bits.argtypes = [vp, C.c_uint32, P(C.c_uint32)]
     lib.dmx_batch_energies.argtypes = [vp, P(C.c_double)]
     lib.dmx_batch_get_dm.argtypes = [vp, C.c_uint32, P(C.c_double), P(C.c_double)]
     lib.dmx_batch_stats.argtypes = [vp, P(C.c_double), P(C.c_uint32), P(C.c_uint32),
@@ -233,6 +258,108 @@ class Sim:
         _check(load().dmx_last_run_stats(self._h, C.byref(ms), C.byref(la), C.byref(sw)))
         return {"ms": ms.value, "launches": la.value, "sweeps": sw.value}
 
+    def set_cost_model(self, model):
+        """Planner model for the next fused upload (None = greedy plan)."""
+        self._model = model  # keep it alive while this simulation may plan with it
+        _check(load().dmx_set_cost_model(self._h, model._h if model is not None else None))
+
+    def set_tile_bits(self, m: int):
+        """Fixed fused tile width 4..8 for the next upload (0 = automatic)."""
+        _check(load().dmx_set_tile_bits(self._h, m))
+
+    def plan_info(self):
+        fb, pb, wg = C.c_uint32(), C.c_uint32(), C.c_uint32()
+        pred, greedy = C.c_double(), C.c_double()
+        tiles = (C.c_uint32 * 5)()
+        _check(load().dmx_plan_info(self._h, C.byref(fb), C.byref(pb), C.byref(wg),
+                                    C.byref(pred), C.byref(greedy), tiles))
+        return {"fused_blocks": fb.value, "plain_blocks": pb.value, "wide_gates": wg.value,
+                "predicted_us": pred.value, "greedy_us": greedy.value,
+                "tiles": {m: tiles[m - 4] for m in range(4, 9)}}
+
+    def time_launch(self, kind, m=0, qmask=0, target=0, control=-1, slow=0, reps=10):
+        """Average device microseconds of one launch configuration (rho is left
+        transformed; reset it before simulating)."""
+        us = C.c_double()
+        _check(load().dmx_time_launch(self._h, kind, m, qmask, target, control, slow, reps,
+                                      C.byref(us)))
+        return us.value
+
+
+def cost_features(n, kind, m=0, qmask=0, target=0, control=-1, slow=0):
+    """Feature vector (float32[COST_FEATURES]) of one launch configuration."""
+    out = np.empty(COST_FEATURES, dtype=np.float32)
+    _check(load().dmx_cost_features(n, kind, m, qmask, target, control, slow,
+                                    _ptr(out, C.c_float)))
+    return out
+
+
+class CostModel:
+    """The fused planner's launch-time model (``dmx_cost_*``): FP32 MLP,
+    COST_FEATURES -> 64 -> 64 -> 1, trained and evaluated on the GPU."""
+
+    def __init__(self, seed: int = 16):
+        h = C.c_void_p()
+        _check(load().dmx_cost_create(seed, C.byref(h)))
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            load().dmx_cost_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @staticmethod
+    def _f32(a):
+        return np.ascontiguousarray(a, dtype=np.float32)
+
+    def train(self, X, y, epochs=200, batch=64, lr=1e-3, seed=0):
+        X, y = self._f32(X), self._f32(y)
+        loss = C.c_float()
+        _check(load().dmx_cost_train(self._h, len(y), _ptr(X, C.c_float), _ptr(y, C.c_float),
+                                     epochs, batch, lr, seed, C.byref(loss)))
+        return loss.value
+
+    def predict(self, X):
+        X = self._f32(X)
+        out = np.empty(len(X), dtype=np.float32)
+        _check(load().dmx_cost_predict(self._h, len(X), _ptr(X, C.c_float),
+                                       _ptr(out, C.c_float)))
+        return out
+
+    def gradient(self, X, y):
+        X, y = self._f32(X), self._f32(y)
+        out = np.empty(COST_PARAMS, dtype=np.float32)
+        _check(load().dmx_cost_gradient(self._h, len(y), _ptr(X, C.c_float),
+                                        _ptr(y, C.c_float), _ptr(out, C.c_float)))
+        return out
+
+    def get_params(self):
+        p = np.empty(COST_PARAMS, dtype=np.float32)
+        mu, dev = C.c_float(), C.c_float()
+        _check(load().dmx_cost_get_params(self._h, _ptr(p, C.c_float), C.byref(mu),
+                                          C.byref(dev)))
+        return p, mu.value, dev.value
+
+    def set_params(self, params, mean=0.0, dev=1.0):
+        p = self._f32(params)
+        if p.size != COST_PARAMS:
+            raise ValueError(f"expected {COST_PARAMS} parameters")
+        _check(load().dmx_cost_set_params(self._h, _ptr(p, C.c_float), mean, dev))
+
+    def save(self, path):
+        p, mu, dev = self.get_params()
+        np.savez(path, params=p, mean=np.float32(mu), dev=np.float32(dev))
+
+    def load(self, path):
+        with np.load(path) as z:
+            self.set_params(z["params"], float(z["mean"]), float(z["dev"]))
+
 
 class Batch:
     """Many independent circuits run together (``dmx_batch_*``)."""
@@ -272,6 +399,14 @@ class Batch:
 
     def set_graph(self, on: bool):
         _check(load().dmx_batch_set_graph(self._h, 1 if on else 0))
+
+    def set_autotune(self, on: bool):
+        _check(load().dmx_batch_set_autotune(self._h, 1 if on else 0))
+
+    def tile_bits(self, n):
+        m = C.c_uint32()
+        _check(load().dmx_batch_tile_bits(self._h, n, C.byref(m)))
+        return m.value
 
     def run(self):
         _check(load().dmx_batch_run(self._h))

@@ -143,6 +143,82 @@ int dmx_set_profiling(dmx_sim* sim, int on);
 int dmx_profile_get(dmx_sim* sim, uint32_t cls, char* name, size_t cap, uint32_t* calls,
                     double* ms, double* bytes, uint32_t* count);
 
+/* ---- Learned fusion cost model.  Extension with no counterpart in the
+ * reference (its planner-free design applies every gate as two sweeps): an
+ * FP32 multilayer perceptron, DMX_COST_FEATURES -> 64 -> 64 -> 1 with ReLU,
+ * trained and evaluated by HIP kernels, that predicts ln(microseconds) of one
+ * kernel launch.  The fused planner prices every block as a fused sweep at
+ * each tile width and as its gates' primitive sweeps, and keeps the cheapest;
+ * without a model it fuses every block at the default width (greedy). ---- */
+#define DMX_COST_FEATURES 12
+#define DMX_COST_PARAMS 5057
+
+/* Launch kinds the model prices. */
+enum dmx_cost_kind {
+    DMX_COST_FUSED = 0, /* dense 4-qubit block, both indices, one sweep */
+    DMX_COST_DENSE = 1, /* 2x2 primitive rewriting both entries, one index */
+    DMX_COST_DIAG = 2,  /* diagonal primitive (only the bit-set entry changes) */
+    DMX_COST_CX = 3,    /* controlled swap */
+    DMX_COST_QUAD = 4   /* 4x4 primitive */
+};
+
+typedef struct dmx_cost dmx_cost;
+
+int dmx_cost_create(uint32_t seed, dmx_cost** out);
+int dmx_cost_destroy(dmx_cost* model);
+
+/* Feature vector (DMX_COST_FEATURES floats) of one launch on n qubits.  fused:
+ * m = super-tile column bits (4..min(8, n)), qmask = the block's four qubits;
+ * sweeps: target (4x4: high qubit), control (cx control, 4x4 low qubit, else
+ * -1), slow = 1 for the slow (row) index.  Host only. */
+int dmx_cost_features(uint32_t n_qubits, int32_t kind, int32_t m, uint32_t qmask, int32_t target,
+                      int32_t control, int32_t slow, float* out);
+
+/* Fits the model to targets y[count] (ln microseconds) from features
+ * X[count][DMX_COST_FEATURES]: targets are standardised by their mean and
+ * deviation, then `epochs` passes of shuffled minibatches of `batch` with Adam
+ * at learning rate lr.  final_loss (may be NULL) = mean squared error on the
+ * standardised targets after training. */
+int dmx_cost_train(dmx_cost* model, uint32_t count, const float* X, const float* y,
+                   uint32_t epochs, uint32_t batch, float lr, uint32_t seed, float* final_loss);
+
+/* out[count] = predictions in the targets' scale. */
+int dmx_cost_predict(dmx_cost* model, uint32_t count, const float* X, float* out);
+
+/* out[DMX_COST_PARAMS] = gradient of mean((f(X) - y)^2) over the parameters
+ * of the raw network (no target standardisation). */
+int dmx_cost_gradient(dmx_cost* model, uint32_t count, const float* X, const float* y,
+                      float* out);
+
+/* Parameters (DMX_COST_PARAMS floats: W1[64][F], b1[64], W2[64][64], b2[64],
+ * w3[64], b3) and the target standardisation (mean, deviation). */
+int dmx_cost_get_params(const dmx_cost* model, float* params, float* mean, float* dev);
+int dmx_cost_set_params(dmx_cost* model, const float* params, float mean, float dev);
+
+/* Planner model of a simulation (NULL = greedy plan); used from the next
+ * dmx_upload with DMX_METHOD_FUSED.  The model must outlive that upload. */
+int dmx_set_cost_model(dmx_sim* sim, dmx_cost* model);
+
+/* Fixed super-tile width m (4..8) for every fused block that admits it
+ * (m <= n; m = 4 only for blocks holding qubit 0), or 0 = automatic (the cost
+ * model's choice, else min(8, n)).  Used from the next dmx_upload; overrides
+ * the model's width choice and its fused-or-plain choice. */
+int dmx_set_tile_bits(dmx_sim* sim, uint32_t m);
+
+/* What the last upload planned: fused blocks, blocks run as primitive sweeps,
+ * gates wider than a block, the model's predicted launch time of one run and
+ * its prediction for the greedy plan (microseconds; 0 without a model), and
+ * tiles[5] = fused blocks per tile width m = 4..8 (may be NULL). */
+int dmx_plan_info(const dmx_sim* sim, uint32_t* fused_blocks, uint32_t* plain_blocks,
+                  uint32_t* wide_gates, double* predicted_us, double* greedy_us, uint32_t* tiles);
+
+/* Timing-data generator: average device time (microseconds) of reps
+ * back-to-back launches of one kernel configuration (arguments as in
+ * dmx_cost_features) on this simulation's rho, after one untimed launch.
+ * rho is left transformed: reset it before simulating. */
+int dmx_time_launch(dmx_sim* sim, int32_t kind, int32_t m, uint32_t qmask, int32_t target,
+                    int32_t control, int32_t slow, uint32_t reps, double* us);
+
 /* ---- Batched executor.  Extension with no counterpart in the reference: many
  * independent circuits of uneven width (4..14 qubits) and depth on one GPU,
  * every rho resident in one device pool, each run a single hipGraph launch
@@ -166,6 +242,14 @@ int dmx_batch_upload(dmx_batch* b);
 
 /* Replay the captured graph (on = 1, default) or enqueue the launches eagerly. */
 int dmx_batch_set_graph(dmx_batch* b, int on);
+
+/* Tile-width autotuning at upload (on = 1, default): for each qubit count in
+ * the batch, the super-tile width (5..min(8, n)) whose batched block launch is
+ * fastest on this GPU; on = 0 keeps min(8, n).  Call before dmx_batch_upload. */
+int dmx_batch_set_autotune(dmx_batch* b, int on);
+
+/* The super-tile width the batch uses for circuits of n qubits (0: none). */
+int dmx_batch_tile_bits(const dmx_batch* b, uint32_t n_qubits, uint32_t* m);
 
 /* Every circuit from |0..0><0..0|; energies become available. */
 int dmx_batch_run(dmx_batch* b);

@@ -78,10 +78,21 @@ def test_batch_matches_oracle_dense(oracle):
         assert max(np.abs(gre - re).max(), np.abs(gim - im).max()) < 1e-10
 
 
-def test_batch_bitwise_equals_single_fused_run():
+@pytest.mark.parametrize("autotune", [True, False])
+def test_batch_bitwise_equals_single_fused_run(autotune):
+    """Tile widths only regroup the 16 x 16 tiles: the batch, autotuned or
+    not, reproduces the single-circuit fused run bit for bit."""
     import dmsim_amd
     items = small_items(seed=9, count=6)
-    b = make_batch(items)
+    b = dmsim_amd.Batch()
+    b.set_autotune(autotune)
+    for circ, strings, weights in items:
+        b.add_circuit(circ, strings, weights)
+    b.upload()
+    for circ, _, _ in items:
+        m = b.tile_bits(circ.n_qubits)
+        n = circ.n_qubits
+        assert (5 if n > 4 else 4) <= m <= min(8, n) and (autotune or m == min(8, n))
     b.run()
     for cid, (circ, _, _) in enumerate(items):
         s = dmsim_amd.Sim(circ.n_qubits)

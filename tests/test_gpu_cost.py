@@ -1,0 +1,163 @@
+"""GPU checks of the learned fusion cost model: the HIP network kernels against
+the NumPy restatement (oracle/mlp_oracle.py), training, the timing-data
+generator, and the learned planner judged by unfused (plain) execution for
+both numerics and launch-time predictions."""
+import math
+import os
+import sys
+
+import numpy as np
+import pytest
+
+from oracle import mlp_oracle as mo
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def dm():
+    import dmsim_amd
+    return dmsim_amd
+
+
+@pytest.fixture(scope="module")
+def costdata():
+    import costdata
+    return costdata
+
+
+def _params(seed, scale=0.2):
+    return np.random.default_rng(seed).normal(0.0, scale, mo.P).astype(np.float32)
+
+
+def test_forward_matches_oracle(dm):
+    print("cost: forward vs oracle", flush=True)
+    p = _params(1)
+    model = dm.CostModel(1)
+    model.set_params(p, 0.0, 1.0)
+    X = np.random.default_rng(2).uniform(0, 1, (777, mo.F)).astype(np.float32)
+    got = model.predict(X)
+    ref = mo.forward(p.astype(np.float64), X.astype(np.float64))
+    # FP32 sums of at most 64 terms: relative error ~1e-6
+    assert np.max(np.abs(got - ref)) <= 1e-4 * max(1.0, np.max(np.abs(ref)))
+    # the de-standardisation is affine in the stored mean and deviation
+    model.set_params(p, 2.0, 3.0)
+    assert np.allclose(model.predict(X), 3.0 * got + 2.0, rtol=1e-6, atol=1e-5)
+
+
+@pytest.mark.parametrize("count", [1, 63, 64, 1000])
+def test_gradient_matches_oracle(dm, count):
+    print(f"cost: gradient vs oracle, count={count}", flush=True)
+    p = _params(3)
+    model = dm.CostModel(3)
+    model.set_params(p, 0.0, 1.0)
+    rng = np.random.default_rng(count)
+    X = rng.uniform(0, 1, (count, mo.F)).astype(np.float32)
+    y = rng.normal(0, 1, count).astype(np.float32)
+    g = model.gradient(X, y)
+    ref = mo.loss_grad(p.astype(np.float64), X.astype(np.float64), y.astype(np.float64))
+    assert np.linalg.norm(g - ref) <= 1e-4 * np.linalg.norm(ref)
+    assert np.max(np.abs(g - ref)) <= 1e-4 * np.max(np.abs(ref))
+    # a second call starts from a zeroed accumulator (atomic order may differ)
+    assert np.allclose(model.gradient(X, y), g, rtol=1e-5, atol=1e-7)
+
+
+def test_adam_step_matches_oracle(dm):
+    print("cost: one Adam step vs oracle", flush=True)
+    p = _params(5)
+    model = dm.CostModel(5)
+    model.set_params(p, 0.0, 1.0)
+    rng = np.random.default_rng(6)
+    X = rng.uniform(0, 1, (256, mo.F)).astype(np.float32)
+    y = rng.normal(0, 1, 256).astype(np.float32)
+    lr = 1e-3
+    model.train(X, y, epochs=1, batch=256, lr=lr, seed=0)  # one full-batch step
+    got, mu, dev = model.get_params()
+    ts = (y.astype(np.float64) - mu) / dev
+    g = mo.loss_grad(p.astype(np.float64), X.astype(np.float64), ts)
+    ref, _, _ = mo.adam_step(p.astype(np.float64), g, np.zeros(mo.P), np.zeros(mo.P), lr, 1)
+    clear = np.abs(g) > 1e-3 * np.max(np.abs(g))  # sign of g unambiguous in FP32
+    assert np.max(np.abs(got - ref)[clear]) <= 1e-6
+    assert np.max(np.abs(got - p)) <= lr * 1.001
+
+
+def test_training_fits_a_smooth_target(dm):
+    print("cost: training on a synthetic target", flush=True)
+    rng = np.random.default_rng(8)
+    X = rng.uniform(0, 1, (3000, mo.F)).astype(np.float32)
+    y = (1.5 * X[:, 0] - X[:, 3] + np.sin(3 * X[:, 5]) + 2 * X[:, 7] * X[:, 8]).astype(np.float32)
+    model = dm.CostModel(8)
+    before = np.mean((model.predict(X[2500:]) - y[2500:]) ** 2)
+    loss = model.train(X[:2500], y[:2500], epochs=150, batch=64, lr=2e-3, seed=1)
+    after = np.mean((model.predict(X[2500:]) - y[2500:]) ** 2)
+    assert loss < 0.01
+    assert after < 0.01 * np.var(y) and after < 0.01 * before
+
+
+@pytest.fixture(scope="module")
+def fitted(dm, costdata):
+    """Timings of launch configurations at 6..10 qubits and a model fitted to
+    80 % of them."""
+    print("cost: collecting launch timings n=6..10", flush=True)
+    X, y, meta = costdata.collect([6, 7, 8, 9, 10], seed=16)
+    perm = np.random.default_rng(0).permutation(len(y))
+    cut = int(0.8 * len(y))
+    model, loss = costdata.fit(X[perm[:cut]], y[perm[:cut]], seed=16, epochs=400)
+    return model, X[perm[cut:]], y[perm[cut:]], meta
+
+
+def test_timing_predictions_hold_out(fitted, costdata):
+    model, Xt, yt, meta = fitted
+    err = costdata.rel_error(model, Xt, yt)
+    print(f"held-out |t_pred/t - 1|: median {np.median(err):.3f} p90 "
+          f"{np.quantile(err, 0.9):.3f} over {len(yt)}", flush=True)
+    assert np.median(err) < 0.15
+    assert np.quantile(err, 0.9) < 0.5
+
+
+def _time_runs(sim, runs=5):
+    ms = []
+    for _ in range(runs):
+        sim.reset_dm()
+        sim.run()
+        ms.append(sim.last_run_stats()["ms"])
+    return float(np.median(ms))
+
+
+@pytest.mark.parametrize("n", [6, 8, 10])
+def test_learned_plan_judged_by_unfused(dm, fitted, n):
+    import workloads as W
+    model = fitted[0]
+    table = W.random_mixed(n, 300, seed=100 + n)
+    print(f"cost: learned plan vs greedy vs plain, n={n}", flush=True)
+
+    def make(method, mdl=None):
+        s = dm.Sim(n)
+        s.set_method(method)
+        if mdl is not None:
+            s.set_cost_model(mdl)
+        s.append_table(table)
+        s.upload()
+        s.run()
+        return s
+
+    plain = make(dm.METHOD_PLAIN_LAUNCH)
+    greedy = make(dm.METHOD_FUSED)
+    learned = make(dm.METHOD_FUSED, model)
+    re0, im0 = plain.get_dm()
+    for s in (greedy, learned):
+        re, im = s.get_dm()
+        assert max(np.abs(re - re0).max(), np.abs(im - im0).max()) <= 1e-10
+    info = learned.plan_info()
+    assert info["predicted_us"] > 0 and info["greedy_us"] > 0
+    assert info["fused_blocks"] + info["plain_blocks"] == greedy.plan_info()["fused_blocks"]
+    t_greedy = _time_runs(greedy)
+    t_learned = _time_runs(learned)
+    print(f"n={n}: greedy {t_greedy:.3f} ms (predicted {info['greedy_us'] / 1e3:.3f}), learned "
+          f"{t_learned:.3f} ms (predicted {info['predicted_us'] / 1e3:.3f}), plan {info}",
+          flush=True)
+    # the model's estimates of both plans against their measured run times
+    assert abs(info["predicted_us"] / 1e3 / t_learned - 1.0) < 0.35
+    assert abs(info["greedy_us"] / 1e3 / t_greedy - 1.0) < 0.35
+    # the learned plan is not slower than greedy beyond timing noise
+    assert t_learned <= 1.10 * t_greedy

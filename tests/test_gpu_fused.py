@@ -59,6 +59,24 @@ def test_fused_random_vs_oracle(oracle, n, max_k):
     check_close(s, re, im, n)
 
 
+@pytest.mark.parametrize("n,m", [(6, 4), (6, 5), (8, 4), (8, 6), (8, 7), (9, 5), (9, 8)])
+def test_fused_tile_widths_vs_oracle(oracle, n, m):
+    """Every super-tile width the planner may choose; m = 4 applies only to
+    blocks holding qubit 0, the others keep the default width."""
+    gate_set = [g for g in UNITARY_SET if W.ARITY[g] <= 4]
+    t = W.random_mixed(n, 200, 5000 + 10 * n + m, gate_set=gate_set)
+    s = dmx().Sim(n)
+    s.set_method(FUSED)
+    s.set_tile_bits(m)
+    s.append_table(t)
+    s.upload()
+    tiles = s.plan_info()["tiles"]
+    assert tiles[m] > 0 and sum(tiles.values()) == s.plan_info()["fused_blocks"]
+    s.run()
+    re, im = oracle.sim(n, t.arrays())
+    check_close(s, re, im, n)
+
+
 def test_fused_non_unitary_r_gate(oracle):
     """R(theta) is linear but not unitary; the fused matrix keeps it."""
     n = 5
