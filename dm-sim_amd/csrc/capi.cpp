@@ -170,6 +170,16 @@ int dmx_get_dm(dmx_sim* sim, double* re, double* im)
     return guarded([&] { return sim->eng.get_dm(re, im); });
 }
 
+int dmx_get_entries(dmx_sim* sim, uint64_t count, const uint64_t* idx, double* re, double* im)
+{
+    if (!sim) return null_handle();
+    if (count && !idx) {
+        dmx::set_error("null index array");
+        return DMX_EINVAL;
+    }
+    return guarded([&] { return sim->eng.get_entries(count, idx, re, im); });
+}
+
 int dmx_get_diag(dmx_sim* sim, double* diag)
 {
     if (!sim) return null_handle();
@@ -387,13 +397,13 @@ int dmx_set_tile_bits(dmx_sim* sim, uint32_t m)
 }
 
 int dmx_plan_info(const dmx_sim* sim, uint32_t* fused_blocks, uint32_t* plain_blocks,
-                  uint32_t* wide_gates, double* predicted_us, double* greedy_us, uint32_t* tiles)
+                  uint32_t* fused5_blocks, double* predicted_us, double* greedy_us, uint32_t* tiles)
 {
     if (!sim) return null_handle();
     const dmx::PlanInfo& pi = sim->eng.plan_info();
     if (fused_blocks) *fused_blocks = pi.fused_blocks;
     if (plain_blocks) *plain_blocks = pi.plain_blocks;
-    if (wide_gates) *wide_gates = pi.wide_gates;
+    if (fused5_blocks) *fused5_blocks = pi.fused5_blocks;
     if (predicted_us) *predicted_us = pi.predicted_us;
     if (greedy_us) *greedy_us = pi.greedy_us;
     if (tiles)

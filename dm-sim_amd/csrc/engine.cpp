@@ -310,6 +310,33 @@ int Engine::get_dm(double* re, double* im)
     return DMX_OK;
 }
 
+int Engine::get_entries(uint64_t count, const uint64_t* idx, double* re, double* im)
+{
+    for (uint64_t t = 0; t < count; t++)
+        if (idx[t] >= num_) {
+            set_error("entry index out of range (must be < 4^n)");
+            return DMX_EINVAL;
+        }
+    if (count == 0) return DMX_OK;
+    void* buf = nullptr;  // indices, then the two output planes
+    HIPTRY(hipMalloc(&buf, count * (sizeof(uint64_t) + 2 * sizeof(double))));
+    uint64_t* d_idx = static_cast<uint64_t*>(buf);
+    double* d_re = reinterpret_cast<double*>(d_idx + count);
+    double* d_im = d_re + count;
+    hipError_t e = hipMemcpyAsync(d_idx, idx, count * sizeof(uint64_t), hipMemcpyHostToDevice,
+                                  stream_);
+    if (e == hipSuccess)
+        e = launch_gather(d_re_, d_im_, n_, d_idx, count, transposed_, d_re, d_im, stream_);
+    if (e == hipSuccess && re)
+        e = hipMemcpyAsync(re, d_re, count * sizeof(double), hipMemcpyDeviceToHost, stream_);
+    if (e == hipSuccess && im)
+        e = hipMemcpyAsync(im, d_im, count * sizeof(double), hipMemcpyDeviceToHost, stream_);
+    if (e == hipSuccess) e = hipStreamSynchronize(stream_);
+    (void)hipFree(buf);
+    HIPTRY(e);
+    return DMX_OK;
+}
+
 int Engine::expect_z(double* out)
 {
     std::vector<double> diag(dim_);

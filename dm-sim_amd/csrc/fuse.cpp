@@ -4,10 +4,12 @@
 // when the union of qubits stays within the width limit and no later block
 // touches its qubits (so moving it there only reorders gates on disjoint
 // qubits, which commute).  Among the admissible blocks the one that grows the
-// least is chosen.  Every block is then padded to four qubits with the lowest
-// qubits it does not use (identity there) and its 16 x 16 matrix is the
-// product of the member gates' matrices, each composed from the gate's own
-// primitive sweeps (so C4X, C3SQRTX, R keep the reference's semantics).
+// least is chosen.  A block of at most four qubits is padded to four with the
+// lowest qubits it does not use (identity there); a five-qubit block (a
+// five-qubit gate such as C4X, or max_k = 5) runs through the 32 x 32 kernel.
+// A block's matrix is the product of the member gates' matrices, each
+// composed from the gate's own primitive sweeps (so C4X, C3SQRTX, R keep the
+// reference's semantics).
 //
 // Execution choice: without a cost model every block runs as one fused sweep
 // at the default tile width.  With one, every block is priced (one batched
@@ -36,25 +38,29 @@ using cd = std::complex<double>;
 // of primitive sweeps (a gate wider than the block width, or a block the cost
 // model runs unfused), applied to the fast index and then to the slow index
 // (valid for complex-linear gates, whose row and column actions commute).
+enum StepType { ST_FUSED4 = 0, ST_PLAIN = 1, ST_FUSED5 = 2 };
+
 struct Step {
-    bool fused;
-    int index;  // block index, or first primitive of the segment
+    int type;
+    int index;  // block index (blocks or blocks5), or first primitive of the segment
     int count;  // primitives in the segment
 };
 
 struct Plan : FusedPlan {
     int n = 0;
     std::vector<FusedBlock> blocks;
-    std::vector<uint32_t> gates_per_block;
+    std::vector<FusedBlock5> blocks5;
     std::vector<PrimOp> prims;
     std::vector<Step> steps;
     FusedBlock* d_blocks = nullptr;
+    FusedBlock5* d_blocks5 = nullptr;
     PrimOp* d_prims = nullptr;
     int max_blocks = kFusedGridCap;
 
     ~Plan() override
     {
         if (d_blocks) (void)hipFree(d_blocks);
+        if (d_blocks5) (void)hipFree(d_blocks5);
         if (d_prims) (void)hipFree(d_prims);
     }
 
@@ -79,6 +85,7 @@ struct Plan : FusedPlan {
     int upload(hipStream_t stream) override
     {
         int rc = put(&d_blocks, blocks, stream);
+        if (!rc) rc = put(&d_blocks5, blocks5, stream);
         return rc ? rc : put(&d_prims, prims, stream);
     }
 
@@ -89,9 +96,15 @@ struct Plan : FusedPlan {
         uint32_t la = 0;
         for (const Step& st : steps) {
             hipError_t e = hipSuccess;
-            if (st.fused) {
+            if (st.type == ST_FUSED4) {
                 prof.begin(KC_FUSED4, sweep_bytes);
                 e = launch_fused4(re, im, d_blocks + st.index, nq, (int)blocks[st.index].m,
+                                  max_blocks, stream);
+                prof.end();
+                la++;
+            } else if (st.type == ST_FUSED5) {
+                prof.begin(KC_FUSED5, sweep_bytes);
+                e = launch_fused5(re, im, d_blocks5 + st.index, nq, (int)blocks5[st.index].m,
                                   max_blocks, stream);
                 prof.end();
                 la++;
@@ -124,11 +137,13 @@ uint32_t gate_mask(const Gate& g)
     return m;
 }
 
-// Left-multiplies the 16x16 block matrix U by gate G embedded on the block's
-// local bits.  G's index bit (kg-1-j) belongs to qubit gq[j].
+// Left-multiplies the D x D block matrix U (D = 2^popcount(qmask)) by gate G
+// embedded on the block's local bits.  G's index bit (kg-1-j) belongs to
+// qubit gq[j].
 void apply_embedded(std::vector<cd>& U, uint32_t qmask, const std::vector<uint32_t>& gq,
                     const std::vector<double>& gre, const std::vector<double>& gim)
 {
+    const int D = 1 << __builtin_popcount(qmask);
     int local[32];
     {
         int t = 0;
@@ -145,15 +160,15 @@ void apply_embedded(std::vector<cd>& U, uint32_t qmask, const std::vector<uint32
             if (x & (1 << local[gq[jj]])) s |= 1 << (kg - 1 - jj);
         return s;
     };
-    std::vector<cd> out(256, 0.0);
-    for (int xo = 0; xo < 16; xo++) {
+    std::vector<cd> out((size_t)D * D, 0.0);
+    for (int xo = 0; xo < D; xo++) {
         const int so = sub(xo);
-        for (int xi = 0; xi < 16; xi++) {
+        for (int xi = 0; xi < D; xi++) {
             if ((xo & ~gl) != (xi & ~gl)) continue;
             const int si = sub(xi);
             const cd gv(gre[(size_t)so * dg + si], gim[(size_t)so * dg + si]);
             if (gv == 0.0) continue;
-            for (int c = 0; c < 16; c++) out[(size_t)xo * 16 + c] += gv * U[(size_t)xi * 16 + c];
+            for (int c = 0; c < D; c++) out[(size_t)xo * D + c] += gv * U[(size_t)xi * D + c];
         }
     }
     U.swap(out);
@@ -166,42 +181,31 @@ int plan_blocks(const std::vector<Gate>& circuit, int n, int max_k, BlockPlan& o
     out = BlockPlan{};
     fusable = false;
     if (n < 4) return DMX_OK;  // too few qubits for a 4-qubit block: plain sweeps
-    const int kmax = std::min(std::max(max_k, 1), 4);
-    for (const Gate& g : circuit) {
+    const int kmax = std::min(std::max(max_k, 1), std::min(5, n));
+    for (const Gate& g : circuit)
         if (g.op == OP_SRN) return DMX_OK;  // not complex-linear: plain sweeps
-        if (g.op == OP_MAT && g.k > 4) {
-            set_error("the fused executor takes matrix gates on at most 4 qubits");
-            return DMX_EINVAL;
-        }
-    }
     fusable = true;
-    // blocks[frozen..] may still take gates; a gate wider than 4 qubits freezes all
     struct PB {
         uint32_t mask;
         std::vector<int> gates;
     };
     std::vector<PB> pbs;
-    int frozen = 0;
     for (int gi = 0; gi < (int)circuit.size(); gi++) {
         const Gate& g = circuit[gi];
         if (g.op == OP_ID) continue;
         const uint32_t S = gate_mask(g);
-        const int k_here = __builtin_popcount(S);
-        if (k_here > 4) {
-            out.order.push_back({1, gi});
-            frozen = (int)pbs.size();
-            continue;
-        }
-        int jstar = frozen - 1;
-        for (int b = (int)pbs.size() - 1; b >= frozen; b--)
+        const int k_here = __builtin_popcount(S);  // <= 5: widest gate or matrix
+        int jstar = -1;  // last block touching the gate's qubits
+        for (int b = (int)pbs.size() - 1; b >= 0; b--)
             if (pbs[b].mask & S) {
                 jstar = b;
                 break;
             }
         int best = -1, best_w = 99;
-        for (int b = std::max(jstar, frozen); b < (int)pbs.size(); b++) {
+        for (int b = std::max(jstar, 0); b < (int)pbs.size(); b++) {
+            const int have = __builtin_popcount(pbs[b].mask);
             const int w = __builtin_popcount(pbs[b].mask | S);
-            if (w <= std::max(kmax, k_here) && w <= best_w) {
+            if (w <= std::max(std::max(kmax, k_here), have) && w <= best_w) {
                 best = b;
                 best_w = w;
             }
@@ -210,7 +214,6 @@ int plan_blocks(const std::vector<Gate>& circuit, int n, int max_k, BlockPlan& o
             pbs[best].mask |= S;
             pbs[best].gates.push_back(gi);
         } else {
-            out.order.push_back({0, (int)pbs.size()});
             pbs.push_back({S, {gi}});
         }
     }
@@ -218,8 +221,9 @@ int plan_blocks(const std::vector<Gate>& circuit, int n, int max_k, BlockPlan& o
     for (const PB& pb : pbs) {
         uint32_t qmask = pb.mask;
         for (int q = 0; q < n && __builtin_popcount(qmask) < 4; q++) qmask |= 1u << q;
-        std::vector<cd> U(256, 0.0);
-        for (int i = 0; i < 16; i++) U[(size_t)i * 16 + i] = 1.0;
+        const int D = 1 << __builtin_popcount(qmask);
+        std::vector<cd> U((size_t)D * D, 0.0);
+        for (int i = 0; i < D; i++) U[(size_t)i * D + i] = 1.0;
         for (int gi : pb.gates) {
             int k = 0;
             std::vector<uint32_t> gq;
@@ -227,17 +231,32 @@ int plan_blocks(const std::vector<Gate>& circuit, int n, int max_k, BlockPlan& o
             gate_matrix(circuit[gi], k, gq, gre, gim);
             apply_embedded(U, qmask, gq, gre, gim);
         }
-        FusedBlock fb;
-        std::memset(&fb, 0, sizeof(fb));
-        fb.qmask = qmask & all;
-        fb.m = (uint32_t)default_tile_bits(n);
-        for (int i = 0; i < 256; i++) {
-            fb.ur[i] = U[i].real();
-            fb.ui[i] = U[i].imag();
+        if (D == 16) {
+            FusedBlock fb;
+            std::memset(&fb, 0, sizeof(fb));
+            fb.qmask = qmask & all;
+            fb.m = (uint32_t)default_tile_bits(n);
+            for (int i = 0; i < 256; i++) {
+                fb.ur[i] = U[i].real();
+                fb.ui[i] = U[i].imag();
+            }
+            out.order.push_back({0, (int)out.blocks.size()});
+            out.blocks.push_back(fb);
+            out.members.push_back(pb.gates);
+        } else {
+            FusedBlock5 fb;
+            std::memset(&fb, 0, sizeof(fb));
+            fb.qmask = qmask & all;
+            fb.m = (uint32_t)default_tile_bits5(n);
+            if (!tile_bits5_ok(n, (int)fb.m, fb.qmask)) fb.m = (uint32_t)std::min(8, n);
+            for (int i = 0; i < 1024; i++) {
+                fb.ur[i] = U[i].real();
+                fb.ui[i] = U[i].imag();
+            }
+            out.order.push_back({2, (int)out.blocks5.size()});
+            out.blocks5.push_back(fb);
+            out.members5.push_back(pb.gates);
         }
-        out.blocks.push_back(fb);
-        out.gates_per_block.push_back((uint32_t)pb.gates.size());
-        out.members.push_back(pb.gates);
     }
     return DMX_OK;
 }
@@ -343,6 +362,8 @@ int build_fused_plan(const std::vector<Gate>& circuit, int n, int max_k, CostMod
     if (tile_bits > 0) {
         for (FusedBlock& fb : bp.blocks)
             if (tile_bits_ok(n, tile_bits, fb.qmask)) fb.m = (uint32_t)tile_bits;
+        for (FusedBlock5& fb : bp.blocks5)
+            if (tile_bits5_ok(n, tile_bits, fb.qmask)) fb.m = (uint32_t)tile_bits;
     } else if (model) {
         rc = choose_executions(circuit, n, bp, *model, plain, pi);
         if (rc) return rc;
@@ -350,22 +371,20 @@ int build_fused_plan(const std::vector<Gate>& circuit, int n, int max_k, CostMod
     Plan* plan = new Plan();
     plan->n = n;
     plan->blocks = std::move(bp.blocks);
-    plan->gates_per_block = std::move(bp.gates_per_block);
+    plan->blocks5 = std::move(bp.blocks5);
     for (const auto& o : bp.order) {
-        const int first = (int)plan->prims.size();
-        if (o.first == 0 && plain[o.second].empty()) {
-            plan->steps.push_back({true, o.second, 0});
+        if (o.first == 2) {
+            plan->steps.push_back({ST_FUSED5, o.second, 0});
+            pi.fused5_blocks++;
+        } else if (plain[o.second].empty()) {
+            plan->steps.push_back({ST_FUSED4, o.second, 0});
             pi.fused_blocks++;
             pi.tile_hist[plan->blocks[o.second].m]++;
-            continue;
-        }
-        if (o.first == 0) {
-            plan->prims.insert(plan->prims.end(), plain[o.second].begin(), plain[o.second].end());
         } else {
-            lower_gate(circuit[o.second], plan->prims);
-            pi.wide_gates++;
+            const int first = (int)plan->prims.size();
+            plan->prims.insert(plan->prims.end(), plain[o.second].begin(), plain[o.second].end());
+            plan->steps.push_back({ST_PLAIN, first, (int)plan->prims.size() - first});
         }
-        plan->steps.push_back({false, first, (int)plan->prims.size() - first});
     }
     *out = plan;
     if (info) *info = pi;

@@ -21,16 +21,19 @@ struct FusedPlan {
                         uint32_t* launches, uint32_t* sweeps) = 0;
 };
 
-// Blocks of a circuit, in execution order interleaved (order) with the gates
-// that are too wide for a block: order[i] = (0, block index) or (1, gate index).
+// Blocks of a circuit in execution order: order[i] = (0, index into blocks)
+// for a block of at most four qubits (padded to four), (2, index into
+// blocks5) for a five-qubit block.
 struct BlockPlan {
     std::vector<FusedBlock> blocks;
-    std::vector<uint32_t> gates_per_block;
     std::vector<std::vector<int>> members;  // circuit indices of each block's gates, in order
+    std::vector<FusedBlock5> blocks5;
+    std::vector<std::vector<int>> members5;
     std::vector<std::pair<int, int>> order;
 };
 
-// Greedy planning into <= max_k-qubit blocks (padded to 4).  fusable = false
+// Greedy planning into blocks of at most max(max_k, widest gate) qubits
+// (max_k <= 5; blocks below four qubits are padded to four).  fusable = false
 // (and no blocks) when the circuit must run the plain method (SRN, n < 4).
 int plan_blocks(const std::vector<Gate>& circuit, int n, int max_k, BlockPlan& out, bool& fusable);
 
@@ -40,9 +43,9 @@ class CostModel;  // mlp.hpp
 // run's launches (0 without a model); greedy_us its estimate for every block
 // fused at the default tile width.
 struct PlanInfo {
-    uint32_t fused_blocks = 0;
-    uint32_t plain_blocks = 0;   // blocks run as primitive sweeps on both indices
-    uint32_t wide_gates = 0;     // gates wider than a block (primitive sweeps)
+    uint32_t fused_blocks = 0;   // four-qubit blocks run fused
+    uint32_t plain_blocks = 0;   // four-qubit blocks run as primitive sweeps on both indices
+    uint32_t fused5_blocks = 0;  // five-qubit blocks (always fused)
     uint32_t tile_hist[9] = {};  // fused blocks per tile width m (4..8)
     double predicted_us = 0.0;
     double greedy_us = 0.0;

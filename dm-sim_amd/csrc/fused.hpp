@@ -34,6 +34,29 @@ inline bool tile_bits_ok(int n, int m, uint32_t qmask)
     return m >= 4 && m <= (n < 8 ? n : 8) && (m > 4 || (qmask & 1u));
 }
 
+// A dense 5-qubit block (32 x 32, the same conventions).
+struct FusedBlock5 {
+    uint32_t qmask;  // exactly five qubits
+    uint32_t m;      // super-tile column bits, see tile_bits5_ok
+    uint32_t pad[2];
+    double ur[1024];
+    double ui[1024];
+};
+
+inline int default_tile_bits5(int n) { return n < 7 ? n : 7; }
+
+// Column bits = the block's five qubits plus the lowest others, m in all;
+// m = 5 needs qubit 0 in the block (column pairs move together).
+inline bool tile_bits5_ok(int n, int m, uint32_t qmask)
+{
+    return m >= 5 && m <= (n < 8 ? n : 8) && (m > 5 || (qmask & 1u));
+}
+
+// rho <- U rho U^dagger for one 5-qubit block (n >= 5), staged in super-tiles
+// of 32 rows x 2^m columns.
+hipError_t launch_fused5(double* re, double* im, const FusedBlock5* d_blk, int n, int m,
+                         int max_blocks, hipStream_t stream);
+
 // rho <- U rho U^dagger for one block (n >= 4), staged in super-tiles of
 // 16 rows x 2^m columns.
 hipError_t launch_fused4(double* re, double* im, const FusedBlock* d_blk, int n, int m,

@@ -14,6 +14,7 @@ enum KernelClass : int {
     KC_CX,            // k_sweep, controlled swap
     KC_QUAD,          // k_quad, 4x4 primitive
     KC_PERSISTENT,    // k_plain_persistent, a segment of primitives
+    KC_FUSED5,        // k_fused5: one dense 5-qubit block, both sides of rho
     KC_COUNT
 };
 

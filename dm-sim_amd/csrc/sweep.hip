@@ -457,6 +457,22 @@ __global__ void k_diag(const double* __restrict__ re, int n, double* __restrict_
         out[i] = re[i * dim + i];
 }
 
+// out[t] = entry idx[t] (linear index r*dim + c of the reference's layout)
+// for t < count; with `transposed` the device holds conj of the transpose.
+__global__ void k_gather(const double* __restrict__ re, const double* __restrict__ im, int n,
+                         const uint64_t* __restrict__ idx, uint64_t count, int transposed,
+                         double* __restrict__ out_re, double* __restrict__ out_im)
+{
+    const uint64_t dim = 1ull << n;
+    for (uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; t < count;
+         t += (uint64_t)gridDim.x * blockDim.x) {
+        uint64_t p = idx[t];
+        if (transposed) p = (p % dim) * dim + p / dim;
+        out_re[t] = re[p];
+        out_im[t] = transposed ? -im[p] : im[p];
+    }
+}
+
 // ---------------------------------------------------------------- host ----
 
 namespace {
@@ -561,6 +577,17 @@ hipError_t launch_conj_transpose(double* re, double* im, int n, hipStream_t stre
     const uint64_t dim = 1ull << n;
     const unsigned t = dim < 32 ? 1u : (unsigned)(dim / 32);
     k_conj_transpose<<<dim3(t, t), 256, 0, stream>>>(re, im, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_gather(const double* re, const double* im, int n, const uint64_t* d_idx,
+                         uint64_t count, bool transposed, double* d_re, double* d_im,
+                         hipStream_t stream)
+{
+    if (count == 0) return hipSuccess;
+    const uint64_t need = (count + 255) / 256;
+    const unsigned blocks = (unsigned)(need < 4096 ? need : 4096);
+    k_gather<<<blocks, 256, 0, stream>>>(re, im, n, d_idx, count, transposed ? 1 : 0, d_re, d_im);
     return hipGetLastError();
 }
 

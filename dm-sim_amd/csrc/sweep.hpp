@@ -38,6 +38,11 @@ inline double prim_bytes(const PrimOp& p, int n)
 // In-place rho <- conj(rho^T).
 hipError_t launch_conj_transpose(double* re, double* im, int n, hipStream_t stream);
 
+// d_re[t], d_im[t] = entry d_idx[t] (< 4^n, reference layout) for t < count.
+hipError_t launch_gather(const double* re, const double* im, int n, const uint64_t* d_idx,
+                         uint64_t count, bool transposed, double* d_re, double* d_im,
+                         hipStream_t stream);
+
 // out[i] = re[i*dim + i].
 hipError_t launch_diag(const double* re, int n, double* out, hipStream_t stream);
 

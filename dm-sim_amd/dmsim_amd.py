@@ -24,6 +24,7 @@ COST_FEATURES, COST_PARAMS = 12, 5057
 EXPORTS = [
     "dmx_create", "dmx_destroy", "dmx_append", "dmx_append_matrix", "dmx_upload", "dmx_run",
     "dmx_clear_circuit", "dmx_reset", "dmx_reset_dm", "dmx_measure", "dmx_get_dm",
+    "dmx_get_entries",
     "dmx_get_diag", "dmx_expect_z", "dmx_info", "dmx_dump", "dmx_set_method",
     "dmx_set_fusion_max_k", "dmx_last_run_stats", "dmx_set_profiling", "dmx_profile_get",
     "dmx_cost_create", "dmx_cost_destroy", "dmx_cost_features", "dmx_cost_train",
@@ -64,6 +65,8 @@ def load():
     lib.dmx_measure.argtypes = [vp, C.c_uint32, C.c_uint32, P(C.c_uint32), P(C.c_double)]
     lib.dmx_get_dm.argtypes = [vp, P(C.c_double), P(C.c_double)]
     lib.dmx_get_diag.argtypes = [vp, P(C.c_double)]
+    lib.dmx_get_entries.argtypes = [vp, C.c_uint64, P(C.c_uint64), P(C.c_double),
+                                    P(C.c_double)]
     lib.dmx_expect_z.argtypes = [vp, P(C.c_double)]
     lib.dmx_info.argtypes = [vp, P(C.c_uint32), P(C.c_uint64), P(C.c_uint32)]
     lib.dmx_dump.argtypes = [vp, C.c_char_p, C.c_size_t, P(C.c_size_t)]
@@ -206,6 +209,16 @@ class Sim:
         _check(load().dmx_get_dm(self._h, _ptr(re, C.c_double), _ptr(im, C.c_double)))
         return re, im
 
+    def get_entries(self, idx):
+        """Entries idx (row-major linear indices, get_dm's convention) as
+        complex numbers, gathered on the device."""
+        idx = np.ascontiguousarray(idx, dtype=np.uint64)
+        re = np.empty(idx.size)
+        im = np.empty(idx.size)
+        _check(load().dmx_get_entries(self._h, idx.size, _ptr(idx, C.c_uint64),
+                                      _ptr(re, C.c_double), _ptr(im, C.c_double)))
+        return re + 1j * im
+
     def get_diag(self):
         d = np.empty(self.dim)
         _check(load().dmx_get_diag(self._h, _ptr(d, C.c_double)))
@@ -273,7 +286,7 @@ class Sim:
         tiles = (C.c_uint32 * 5)()
         _check(load().dmx_plan_info(self._h, C.byref(fb), C.byref(pb), C.byref(wg),
                                     C.byref(pred), C.byref(greedy), tiles))
-        return {"fused_blocks": fb.value, "plain_blocks": pb.value, "wide_gates": wg.value,
+        return {"fused_blocks": fb.value, "plain_blocks": pb.value, "fused5_blocks": wg.value,
                 "predicted_us": pred.value, "greedy_us": greedy.value,
                 "tiles": {m: tiles[m - 4] for m in range(4, 9)}}
 

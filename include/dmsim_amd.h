@@ -45,7 +45,7 @@ enum dmx_op {
 /* Execution methods (dmx_set_method). */
 #define DMX_METHOD_PLAIN 0        /* persistent cooperative gate loop, 2 sweeps per primitive */
 #define DMX_METHOD_PLAIN_LAUNCH 1 /* same sweeps, one launch per primitive                    */
-#define DMX_METHOD_FUSED 2        /* gates fused into dense k<=5 blocks, one sweep per block  */
+#define DMX_METHOD_FUSED 2        /* gates fused into dense 4- and 5-qubit blocks, one sweep each */
 
 typedef struct dmx_sim dmx_sim;
 
@@ -105,6 +105,12 @@ int dmx_measure(dmx_sim* sim, uint32_t repetition, uint32_t seed, uint32_t* out,
  * 4^n doubles each.  Either pointer may be NULL. */
 int dmx_get_dm(dmx_sim* sim, double* re, double* im);
 
+/* Extension: selected entries of dm_real_res / dm_imag_res without copying
+ * the whole matrix (which is 64 GiB per plane at n = 16): re[t], im[t] =
+ * entry idx[t] (row-major linear index < 4^n, same convention as dmx_get_dm).
+ * re or im may be NULL. */
+int dmx_get_entries(dmx_sim* sim, uint64_t count, const uint64_t* idx, double* re, double* im);
+
 /* Re rho_ii, i < 2^n (the diagonal measure() and print_res_sv() read,
  * src/dmsim_amdgpu_omp.hpp:541-543, :565-575). */
 int dmx_get_diag(dmx_sim* sim, double* diag);
@@ -123,7 +129,8 @@ int dmx_dump(const dmx_sim* sim, char* buf, size_t cap, size_t* needed);
 /* Execution method for subsequent run() calls (default DMX_METHOD_PLAIN). */
 int dmx_set_method(dmx_sim* sim, int method);
 
-/* Fusion limit for DMX_METHOD_FUSED: largest block width k (1..5). */
+/* Fusion limit for DMX_METHOD_FUSED: blocks grow to at most k qubits (1..5,
+ * default 4); a gate on five qubits always forms a five-qubit block. */
 int dmx_set_fusion_max_k(dmx_sim* sim, uint32_t k);
 
 /* Device time of the last run() in milliseconds (HIP events on the run's
@@ -205,12 +212,15 @@ int dmx_set_cost_model(dmx_sim* sim, dmx_cost* model);
  * the model's width choice and its fused-or-plain choice. */
 int dmx_set_tile_bits(dmx_sim* sim, uint32_t m);
 
-/* What the last upload planned: fused blocks, blocks run as primitive sweeps,
- * gates wider than a block, the model's predicted launch time of one run and
- * its prediction for the greedy plan (microseconds; 0 without a model), and
- * tiles[5] = fused blocks per tile width m = 4..8 (may be NULL). */
+/* What the last upload planned: four-qubit blocks run fused, four-qubit
+ * blocks run as primitive sweeps (the model's choice), five-qubit blocks, the
+ * model's predicted device time of the four-qubit blocks' launches and its
+ * prediction for the same blocks fused at the default width (microseconds; 0
+ * without a model), and tiles[5] = fused four-qubit blocks per tile width
+ * m = 4..8 (may be NULL). */
 int dmx_plan_info(const dmx_sim* sim, uint32_t* fused_blocks, uint32_t* plain_blocks,
-                  uint32_t* wide_gates, double* predicted_us, double* greedy_us, uint32_t* tiles);
+                  uint32_t* fused5_blocks, double* predicted_us, double* greedy_us,
+                  uint32_t* tiles);
 
 /* Timing-data generator: average device time (microseconds) of reps
  * back-to-back launches of one kernel configuration (arguments as in

@@ -1,5 +1,5 @@
-"""GPU checks of the fused method (dense 4-qubit blocks, FP64 MFMA, both sides
-of rho in one sweep).  Fusing reorders floating-point work, so the bar is
+"""GPU checks of the fused method (dense 4- and 5-qubit blocks, FP64 MFMA, both
+sides of rho in one sweep).  Fusing reorders floating-point work, so the bar is
 max|d rho| <= 1e-10 against the oracle (the reference CPU path), with trace and
 Hermiticity to 1e-12; circuits the planner cannot fuse (SRN, n < 4) fall back
 to the plain sweeps and stay bit-exact."""
@@ -89,15 +89,16 @@ def test_fused_non_unitary_r_gate(oracle):
     assert max(np.abs(got_re - re).max(), np.abs(got_im - im).max()) <= TOL * scale
 
 
-def test_fused_matrix_gates_dense_brute_force():
-    """3- and 4-qubit matrix gates (fused-only) against numpy's conj(M rho M^+)."""
-    n = 5
-    rng = np.random.default_rng(5)
+@pytest.mark.parametrize("n", [5, 6])
+def test_fused_matrix_gates_dense_brute_force(n):
+    """1- to 5-qubit matrix gates (3..5 are fused-only) against numpy's
+    conj(M rho M^+)."""
+    rng = np.random.default_rng(5 + n)
     s = dmx().Sim(n)
     s.set_method(FUSED)
     M = np.eye(1 << n, dtype=complex)
-    for _ in range(6):
-        k = int(rng.integers(1, 5))
+    for _ in range(8):
+        k = int(rng.integers(1, 6))
         qs = [int(x) for x in rng.choice(n, size=k, replace=False)]
         u = W.haar_unitary(rng, 1 << k)
         s.append_matrix(qs, u)
@@ -119,14 +120,47 @@ def test_fused_matrix_gates_dense_brute_force():
     assert np.max(np.abs(got - want)) <= TOL
 
 
-def test_fused_c4x_runs_as_plain_segment(oracle):
-    """C4X spans 5 qubits, above the block width: it runs as a plain segment
-    between fused blocks, keeping the reference's (non-textbook) semantics."""
-    n = 6
-    gate_set = ["H", "CX", "U3", "CCX", "C4X", "RZZ", "C3X"]
-    t = W.random_mixed(n, 40, 4, gate_set=gate_set)
+@pytest.mark.parametrize("n", [5, 6, 8, 9])
+def test_fused_c4x_five_qubit_blocks(oracle, n):
+    """C4X spans five qubits: it forms a 32 x 32 block (k_fused5) composed from
+    the reference's own primitive sequence, so its (non-textbook) semantics
+    carry over; neighbouring gates on its qubits join the block."""
+    gate_set = ["H", "CX", "U3", "CCX", "C4X", "RZZ", "C3X", "CSWAP"]
+    t = W.random_mixed(n, 60, 4 + n, gate_set=gate_set)
     assert W.OP["C4X"] in t.ops
     s = run(n, [t])
+    info = s.plan_info()
+    assert info["fused5_blocks"] >= 1
+    assert s.last_run_stats()["launches"] == info["fused_blocks"] + info["fused5_blocks"]
+    re, im = oracle.sim(n, t.arrays())
+    check_close(s, re, im, n)
+
+
+@pytest.mark.parametrize("n", [5, 6, 8, 9])
+def test_fused_max_k5_random_vs_oracle(oracle, n):
+    """Blocks grown to five qubits (max_k = 5)."""
+    gate_set = [g for g in UNITARY_SET if W.ARITY[g] <= 5]
+    t = W.random_mixed(n, 300, 7000 + n, gate_set=gate_set)
+    s = run(n, [t], max_k=5)
+    assert s.plan_info()["fused5_blocks"] > 0
+    re, im = oracle.sim(n, t.arrays())
+    check_close(s, re, im, n)
+
+
+@pytest.mark.parametrize("n,m", [(6, 5), (6, 6), (8, 5), (8, 6), (8, 7), (9, 8)])
+def test_fused5_tile_widths_vs_oracle(oracle, n, m):
+    """Every super-tile width of the five-qubit kernel; m = 5 applies only to
+    blocks holding qubit 0."""
+    gate_set = [g for g in UNITARY_SET if W.ARITY[g] <= 5]
+    t = W.random_mixed(n, 150, 7100 + 10 * n + m, gate_set=gate_set)
+    s = dmx().Sim(n)
+    s.set_method(FUSED)
+    s.set_fusion_max_k(5)
+    s.set_tile_bits(m)
+    s.append_table(t)
+    s.upload()
+    assert s.plan_info()["fused5_blocks"] > 0
+    s.run()
     re, im = oracle.sim(n, t.arrays())
     check_close(s, re, im, n)
 
