@@ -201,13 +201,14 @@ int plan_blocks(const std::vector<Gate>& circuit, int n, int max_k, BlockPlan& o
                 jstar = b;
                 break;
             }
-        int best = -1, best_w = 99;
+        // least growth (new qubits the block must take), the latest block on ties
+        int best = -1, best_g = 99;
         for (int b = std::max(jstar, 0); b < (int)pbs.size(); b++) {
             const int have = __builtin_popcount(pbs[b].mask);
             const int w = __builtin_popcount(pbs[b].mask | S);
-            if (w <= std::max(std::max(kmax, k_here), have) && w <= best_w) {
+            if (w <= std::max(std::max(kmax, k_here), have) && w - have <= best_g) {
                 best = b;
-                best_w = w;
+                best_g = w - have;
             }
         }
         if (best >= 0) {
