@@ -147,6 +147,36 @@ def batched(steps, warmup, seed=4, n_circuits=256):
     return out
 
 
+def vqe_published(warmup=1, steps=5):
+    """The circuit of the reference's only published single-GPU timing:
+    VQE-UCCSD on 8 qubits, 10 808 gates, 249.3 ms on one V100 (README.md:47),
+    read from OpenQASM by the engine's front-end.  Median device time of a run
+    from |0><0| after a warm-up run, per method."""
+    import dmsim_amd
+
+    path = os.path.join(ROOT, "tests", "golden", "qasm", "vqe_uccsd_n8.qasm")
+    with open(path) as f:
+        src = f.read()
+    out = {"workload": "vqe_uccsd_n8.qasm: 8 qubits, 10808 gates",
+           "reference_ms": 249.3, "reference_hw": "1x NVIDIA V100 (README.md:47)"}
+    for mname in ("fused", "plain"):
+        s = dmsim_amd.Sim(8)
+        s.set_method(METHODS[mname])
+        s.append_qasm(src)
+        s.upload()
+        ms = []
+        for i in range(warmup + steps):
+            s.reset_dm()
+            s.run()
+            if i >= warmup:
+                ms.append(s.last_run_stats()["ms"])
+        s.close()
+        ms.sort()
+        out[f"{mname}_ms"] = ms[len(ms) // 2]
+    out["speedup_vs_reference"] = out["reference_ms"] / min(out["fused_ms"], out["plain_ms"])
+    return out
+
+
 def calibrate_copy():
     """A plain device-to-device copy of 2 GiB per plane pair (4 GiB read +
     4 GiB written in total) for scaling the FETCH_SIZE/WRITE_SIZE counters."""
@@ -271,6 +301,10 @@ def main():
         line["batched"] = batched(args.steps, args.warmup)
         log(f"[bench] batched: {line['batched']['batched_gate_updates_per_s']:.0f} "
             f"gate-updates/s, {line['batched']['batched_vs_plain']:.1f}x one-by-one plain")
+    log("[bench] published comparison (vqe_uccsd_n8) ...")
+    line["published_comparison"] = vqe_published()
+    log(f"[bench] vqe_uccsd_n8: fused {line['published_comparison']['fused_ms']:.2f} ms, "
+        f"plain {line['published_comparison']['plain_ms']:.2f} ms (V100: 249.3 ms)")
     if not args.no_cpu_baseline:
         log("[bench] cpu baseline ...")
         line["cpu_baseline"] = cpu_baseline(table, n)

@@ -10,6 +10,7 @@
 #include "engine.hpp"
 #include "fused.hpp"
 #include "mlp.hpp"
+#include "qasm.hpp"
 
 struct dmx_sim {
     dmx::Engine eng;
@@ -270,6 +271,58 @@ int dmx_profile_get(dmx_sim* sim, uint32_t cls, char* name, size_t cap, uint32_t
     if (ms) *ms = st.ms;
     if (bytes) *bytes = st.bytes;
     return DMX_OK;
+}
+
+int dmx_qasm_parse(const char* source, uint32_t cap, int32_t* ops, uint32_t* qb, double* params,
+                   uint32_t* n_qubits, uint32_t* n_gates)
+{
+    if (!source || (cap && (!ops || !qb || !params))) {
+        dmx::set_error("null source or output arrays");
+        return DMX_EINVAL;
+    }
+    return guarded([&] {
+        dmx::QasmProgram prog;
+        int rc = dmx::parse_qasm(source, prog);
+        if (rc) return rc;
+        const size_t m = prog.gates.size() < cap ? prog.gates.size() : cap;
+        for (size_t i = 0; i < m; i++) {
+            const dmx::Gate& g = prog.gates[i];
+            ops[i] = g.op;
+            for (int j = 0; j < 5; j++) qb[5 * i + j] = g.qb[j];
+            params[3 * i] = g.theta;
+            params[3 * i + 1] = g.phi;
+            params[3 * i + 2] = g.lambda;
+        }
+        if (n_qubits) *n_qubits = (uint32_t)prog.n_qubits;
+        if (n_gates) *n_gates = (uint32_t)prog.gates.size();
+        return DMX_OK;
+    });
+}
+
+int dmx_append_qasm(dmx_sim* sim, const char* source, uint32_t* n_gates)
+{
+    if (!sim) return null_handle();
+    if (!source) {
+        dmx::set_error("null source");
+        return DMX_EINVAL;
+    }
+    return guarded([&] {
+        dmx::QasmProgram prog;
+        int rc = dmx::parse_qasm(source, prog);
+        if (rc) return rc;
+        if ((uint32_t)prog.n_qubits > sim->eng.n_qubits()) {
+            dmx::set_error("the program declares " + std::to_string(prog.n_qubits) +
+                           " qubits; the simulation has " +
+                           std::to_string(sim->eng.n_qubits()));
+            return DMX_EINVAL;
+        }
+        for (const dmx::Gate& g : prog.gates) {
+            rc = sim->eng.append(g);
+            if (rc) return rc;
+        }
+        if (n_gates) *n_gates = (uint32_t)prog.gates.size();
+        return DMX_OK;
+    });
 }
 
 int dmx_cost_create(uint32_t seed, dmx_cost** out)

@@ -27,6 +27,7 @@ EXPORTS = [
     "dmx_get_entries",
     "dmx_get_diag", "dmx_expect_z", "dmx_info", "dmx_dump", "dmx_set_method",
     "dmx_set_fusion_max_k", "dmx_last_run_stats", "dmx_set_profiling", "dmx_profile_get",
+    "dmx_qasm_parse", "dmx_append_qasm",
     "dmx_cost_create", "dmx_cost_destroy", "dmx_cost_features", "dmx_cost_train",
     "dmx_cost_predict", "dmx_cost_gradient", "dmx_cost_get_params", "dmx_cost_set_params",
     "dmx_set_cost_model", "dmx_set_tile_bits", "dmx_plan_info", "dmx_time_launch",
@@ -76,6 +77,9 @@ def load():
     lib.dmx_set_profiling.argtypes = [vp, C.c_int]
     lib.dmx_profile_get.argtypes = [vp, C.c_uint32, C.c_char_p, C.c_size_t, P(C.c_uint32),
                                     P(C.c_double), P(C.c_double), P(C.c_uint32)]
+    lib.dmx_qasm_parse.argtypes = [C.c_char_p, C.c_uint32, P(C.c_int32), P(C.c_uint32),
+                                   P(C.c_double), P(C.c_uint32), P(C.c_uint32)]
+    lib.dmx_append_qasm.argtypes = [vp, C.c_char_p, P(C.c_uint32)]
     f32p = P(C.c_float)
     lib.dmx_cost_create.argtypes = [C.c_uint32, P(vp)]
     lib.dmx_cost_destroy.argtypes = [vp]
@@ -271,6 +275,12 @@ class Sim:
         _check(load().dmx_last_run_stats(self._h, C.byref(ms), C.byref(la), C.byref(sw)))
         return {"ms": ms.value, "launches": la.value, "sweeps": sw.value}
 
+    def append_qasm(self, source: str):
+        """Append an OpenQASM 2.0 program's gates; returns how many."""
+        ng = C.c_uint32()
+        _check(load().dmx_append_qasm(self._h, source.encode(), C.byref(ng)))
+        return ng.value
+
     def set_cost_model(self, model):
         """Planner model for the next fused upload (None = greedy plan)."""
         self._model = model  # keep it alive while this simulation may plan with it
@@ -297,6 +307,21 @@ class Sim:
         _check(load().dmx_time_launch(self._h, kind, m, qmask, target, control, slow, reps,
                                       C.byref(us)))
         return us.value
+
+
+def qasm_parse(source: str):
+    """OpenQASM 2.0 text -> (n_qubits, ops int32[g], qb uint32[g, 5], params float64[g, 3])."""
+    lib = load()
+    src = source.encode()
+    nq, ng = C.c_uint32(), C.c_uint32()
+    _check(lib.dmx_qasm_parse(src, 0, None, None, None, C.byref(nq), C.byref(ng)))
+    g = ng.value
+    ops = np.zeros(max(g, 1), dtype=np.int32)
+    qb = np.zeros((max(g, 1), 5), dtype=np.uint32)
+    prm = np.zeros((max(g, 1), 3))
+    _check(lib.dmx_qasm_parse(src, g, _ptr(ops, C.c_int32), _ptr(qb, C.c_uint32),
+                              _ptr(prm, C.c_double), C.byref(nq), C.byref(ng)))
+    return nq.value, ops[:g], qb[:g], prm[:g]
 
 
 def cost_features(n, kind, m=0, qmask=0, target=0, control=-1, slow=0):

@@ -150,6 +150,28 @@ int dmx_set_profiling(dmx_sim* sim, int on);
 int dmx_profile_get(dmx_sim* sim, uint32_t cls, char* name, size_t cap, uint32_t* calls,
                     double* ms, double* bytes, uint32_t* count);
 
+/* ---- OpenQASM 2.0 front-end.  Replaces the reference's translator
+ * tool/dmsim_qasm.py (Python 2; it writes a Python script of Simulation
+ * factory calls): registers are numbered in declaration order on one qubit
+ * index space (:266-272), qelib1 gate names map to the factories of the same
+ * name with their parameter placement (:29-69), a statement on whole registers
+ * is broadcast over their bits (:133-160), user-defined gates are expanded at
+ * each call (:240-266); measure, barrier, creg, reset and if statements are
+ * accepted and ignored (:105).  Parse errors are DMX_EINVAL with the line in
+ * dmx_last_error(). ---- */
+
+/* Parses `source` into a gate table: the first min(cap, *n_gates) gates go to
+ * ops[i], qb[5*i..5*i+4], params[3*i..3*i+2] (theta, phi, lambda as in
+ * dmx_append); *n_qubits and *n_gates receive the program's totals.  Arrays
+ * may be NULL when cap is 0.  Host only. */
+int dmx_qasm_parse(const char* source, uint32_t cap, int32_t* ops, uint32_t* qb, double* params,
+                   uint32_t* n_qubits, uint32_t* n_gates);
+
+/* Parses `source` and appends its gates to the circuit (as dmx_append would);
+ * the program's qubit count must not exceed the simulation's.  *n_gates (may
+ * be NULL) receives the number appended. */
+int dmx_append_qasm(dmx_sim* sim, const char* source, uint32_t* n_gates);
+
 /* ---- Learned fusion cost model.  Extension with no counterpart in the
  * reference (its planner-free design applies every gate as two sweeps): an
  * FP32 multilayer perceptron, DMX_COST_FEATURES -> 64 -> 64 -> 1 with ReLU,
