@@ -244,3 +244,15 @@ def test_rx_and_deuteron_on_gpu():
         sim.run()
         d = sim.get_diag()
         assert abs((d[0] - d[1] - d[2] + d[3]) - math.sin(tval)) < 1e-13
+
+
+def test_cpp_facade_adder(tmp_path):
+    """The reference's adder example written against include/dmsim_amd.hpp,
+    compiled and run as its own process: every shot reads 1000000010."""
+    import subprocess
+    from test_capi_host import build_cpp_facade_example
+    exe = build_cpp_facade_example(tmp_path)
+    print("cpp facade: running adder_facade", flush=True)
+    r = subprocess.run([exe], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert r.stdout.count("1000000010") == 5
