@@ -121,6 +121,54 @@ int dmx_append_matrix(dmx_sim* sim, uint32_t k, const uint32_t* qubits, const do
     });
 }
 
+int dmx_append_kraus(dmx_sim* sim, uint32_t qubit, uint32_t n_ops, const double* k_re,
+                     const double* k_im)
+{
+    if (!sim) return null_handle();
+    if (n_ops < 1 || n_ops > 64 || !k_re || !k_im) {
+        dmx::set_error("a channel needs 1..64 Kraus operators and non-null arrays");
+        return DMX_EINVAL;
+    }
+    return guarded([&] {
+        // stored A = conj(rho):  A' = sum_i conj(K_i) A K_i^T, so on the entry quad
+        // (row bit a, column bit b) S[(a,b),(a',b')] = sum_i conj(K_i[a][a']) K_i[b][b'];
+        // for the conj-transposed orientation S'[(a,b),(a',b')] = conj(S[(b,a),(b',a')]).
+        double sr[16] = {}, si[16] = {};
+        for (uint32_t i = 0; i < n_ops; i++) {
+            const double* kr = k_re + 4 * i;
+            const double* ki = k_im + 4 * i;
+            for (int a = 0; a < 2; a++)
+                for (int b = 0; b < 2; b++)
+                    for (int a2 = 0; a2 < 2; a2++)
+                        for (int b2 = 0; b2 < 2; b2++) {
+                            const double xr = kr[2 * a + a2], xi = -ki[2 * a + a2];  // conj
+                            const double yr = kr[2 * b + b2], yi = ki[2 * b + b2];
+                            const int idx = (2 * a + b) * 4 + (2 * a2 + b2);
+                            sr[idx] += xr * yr - xi * yi;
+                            si[idx] += xr * yi + xi * yr;
+                        }
+        }
+        dmx::Gate g;
+        g.op = dmx::OP_KRAUS;
+        g.k = 1;
+        g.qb[0] = qubit;
+        g.mre.assign(32, 0.0);
+        g.mim.assign(32, 0.0);
+        for (int a = 0; a < 2; a++)
+            for (int b = 0; b < 2; b++)
+                for (int a2 = 0; a2 < 2; a2++)
+                    for (int b2 = 0; b2 < 2; b2++) {
+                        const int idx = (2 * a + b) * 4 + (2 * a2 + b2);
+                        const int tdx = (2 * b + a) * 4 + (2 * b2 + a2);
+                        g.mre[idx] = sr[idx];
+                        g.mim[idx] = si[idx];
+                        g.mre[16 + idx] = sr[tdx];
+                        g.mim[16 + idx] = -si[tdx];
+                    }
+        return sim->eng.append(g);
+    });
+}
+
 int dmx_upload(dmx_sim* sim)
 {
     if (!sim) return null_handle();

@@ -77,6 +77,10 @@ int Engine::append(const Gate& g)
         return DMX_EINVAL;
     }
     const int k = g.op == OP_MAT ? g.k : op_arity(g.op);
+    if (g.op == OP_KRAUS && (g.mre.size() != 32 || g.mim.size() != 32)) {
+        set_error("channel record needs two 4x4 superoperators");
+        return DMX_EINVAL;
+    }
     if (g.op == OP_MAT) {
         if (k < 1 || k > 5 || (int)g.mre.size() != (1 << (2 * k)) ||
             (int)g.mim.size() != (1 << (2 * k))) {
@@ -175,37 +179,57 @@ int Engine::run_plain(bool per_launch)
 {
     const int n_ops = (int)prims_.size();
     last_launches_ = 0;
-    last_sweeps_ = 2u * (uint32_t)n_ops;
+    last_sweeps_ = 0;
     prof_.start(stream_);
     HIPTRY(hipEventRecord(ev0_, stream_));
-    if (n_ops > 0) {
-        // reference orientation: fast index first; transposed: slow first
-        const bool first_slow = transposed_;
-        for (int pass = 0; pass < 2; pass++) {
-            const bool slow = pass == 0 ? first_slow : !first_slow;
-            int t = 0;
-            while (t < n_ops) {
-                if (per_launch || prims_[t].kind == PK_C2) {
-                    prof_.begin(prim_class(prims_[t]), prim_bytes(prims_[t], n_));
-                    HIPTRY(launch_plain_one(d_re_, d_im_, d_ops_ + t, prims_[t], n_, slow,
-                                            max_blocks_, stream_));
+    // Segments of gate primitives between channels: each segment runs its two
+    // index passes (the reference's forward and backward pass); a channel then
+    // acts once on both indices, in the record for the current orientation.
+    int s0 = 0;
+    while (s0 < n_ops) {
+        int s1 = s0;
+        while (s1 < n_ops && prims_[s1].kind != PK_SUP) s1++;
+        if (s1 > s0) {
+            // reference orientation: fast index first; transposed: slow first
+            const bool first_slow = transposed_;
+            for (int pass = 0; pass < 2; pass++) {
+                const bool slow = pass == 0 ? first_slow : !first_slow;
+                int t = s0;
+                while (t < s1) {
+                    if (per_launch || prims_[t].kind == PK_C2) {
+                        prof_.begin(prim_class(prims_[t]), prim_bytes(prims_[t], n_));
+                        HIPTRY(launch_plain_one(d_re_, d_im_, d_ops_ + t, prims_[t], n_, slow,
+                                                max_blocks_, stream_));
+                        prof_.end();
+                        last_launches_++;
+                        t++;
+                        continue;
+                    }
+                    int end = t;
+                    double bytes = 0;
+                    while (end < s1 && prims_[end].kind != PK_C2)
+                        bytes += prim_bytes(prims_[end++], n_);
+                    prof_.begin(KC_PERSISTENT, bytes);
+                    HIPTRY(launch_plain_persistent(d_re_, d_im_, d_ops_, t, end, n_, slow,
+                                                   persistent_blocks_, stream_));
                     prof_.end();
                     last_launches_++;
-                    t++;
-                    continue;
+                    t = end;
                 }
-                int end = t;
-                double bytes = 0;
-                while (end < n_ops && prims_[end].kind != PK_C2) bytes += prim_bytes(prims_[end++], n_);
-                prof_.begin(KC_PERSISTENT, bytes);
-                HIPTRY(launch_plain_persistent(d_re_, d_im_, d_ops_, t, end, n_, slow,
-                                               persistent_blocks_, stream_));
-                prof_.end();
-                last_launches_++;
-                t = end;
             }
+            last_sweeps_ += 2u * (uint32_t)(s1 - s0);
+            transposed_ = !transposed_;
         }
-        transposed_ = !transposed_;
+        if (s1 < n_ops) {  // channel: records s1 (stored orientation), s1 + 1 (transposed)
+            const int pick = s1 + (transposed_ ? 1 : 0);
+            prof_.begin(KC_QUAD, prim_bytes(prims_[pick], n_));
+            HIPTRY(launch_plain_one(d_re_, d_im_, d_ops_ + pick, prims_[pick], n_, false,
+                                    max_blocks_, stream_));
+            prof_.end();
+            last_launches_++;
+            last_sweeps_++;
+        }
+        s0 = s1 + 2;
     }
     HIPTRY(hipEventRecord(ev1_, stream_));
     HIPTRY(hipEventSynchronize(ev1_));
@@ -222,7 +246,8 @@ int Engine::run_fused()
     last_sweeps_ = 0;
     prof_.start(stream_);
     HIPTRY(hipEventRecord(ev0_, stream_));
-    int rc = plan_->execute(d_re_, d_im_, n_, stream_, prof_, &last_launches_, &last_sweeps_);
+    int rc = plan_->execute(d_re_, d_im_, n_, transposed_, stream_, prof_, &last_launches_,
+                            &last_sweeps_);
     if (rc) return rc;
     HIPTRY(hipEventRecord(ev1_, stream_));
     HIPTRY(hipEventSynchronize(ev1_));

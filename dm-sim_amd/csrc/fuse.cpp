@@ -38,7 +38,7 @@ using cd = std::complex<double>;
 // of primitive sweeps (a gate wider than the block width, or a block the cost
 // model runs unfused), applied to the fast index and then to the slow index
 // (valid for complex-linear gates, whose row and column actions commute).
-enum StepType { ST_FUSED4 = 0, ST_PLAIN = 1, ST_FUSED5 = 2 };
+enum StepType { ST_FUSED4 = 0, ST_PLAIN = 1, ST_FUSED5 = 2, ST_CHANNEL = 3 };
 
 struct Step {
     int type;
@@ -89,8 +89,8 @@ struct Plan : FusedPlan {
         return rc ? rc : put(&d_prims, prims, stream);
     }
 
-    int execute(double* re, double* im, int nq, hipStream_t stream, LaunchProfiler& prof,
-                uint32_t* launches, uint32_t* sweeps) override
+    int execute(double* re, double* im, int nq, bool transposed, hipStream_t stream,
+                LaunchProfiler& prof, uint32_t* launches, uint32_t* sweeps) override
     {
         const double sweep_bytes = 32.0 * (double)(1ull << (2 * nq));
         uint32_t la = 0;
@@ -100,6 +100,13 @@ struct Plan : FusedPlan {
                 prof.begin(KC_FUSED4, sweep_bytes);
                 e = launch_fused4(re, im, d_blocks + st.index, nq, (int)blocks[st.index].m,
                                   max_blocks, stream);
+                prof.end();
+                la++;
+            } else if (st.type == ST_CHANNEL) {
+                const int t = st.index + (transposed ? 1 : 0);  // record for this orientation
+                prof.begin(KC_QUAD, prim_bytes(prims[t], nq));
+                e = launch_plain_one(re, im, d_prims + t, prims[t], nq, false, kSweepGridCap,
+                                     stream);
                 prof.end();
                 la++;
             } else if (st.type == ST_FUSED5) {
@@ -190,20 +197,27 @@ int plan_blocks(const std::vector<Gate>& circuit, int n, int max_k, BlockPlan& o
         std::vector<int> gates;
     };
     std::vector<PB> pbs;
+    std::vector<std::pair<int, int>> seq;  // (0, index into pbs) or (3, channel gate)
+    int frozen = 0;                        // blocks before a channel take no later gate
     for (int gi = 0; gi < (int)circuit.size(); gi++) {
         const Gate& g = circuit[gi];
         if (g.op == OP_ID) continue;
+        if (g.op == OP_KRAUS) {
+            seq.push_back({3, gi});
+            frozen = (int)pbs.size();
+            continue;
+        }
         const uint32_t S = gate_mask(g);
         const int k_here = __builtin_popcount(S);  // <= 5: widest gate or matrix
-        int jstar = -1;  // last block touching the gate's qubits
-        for (int b = (int)pbs.size() - 1; b >= 0; b--)
+        int jstar = frozen - 1;  // last block touching the gate's qubits
+        for (int b = (int)pbs.size() - 1; b >= frozen; b--)
             if (pbs[b].mask & S) {
                 jstar = b;
                 break;
             }
         // least growth (new qubits the block must take), the latest block on ties
         int best = -1, best_g = 99;
-        for (int b = std::max(jstar, 0); b < (int)pbs.size(); b++) {
+        for (int b = std::max(jstar, frozen); b < (int)pbs.size(); b++) {
             const int have = __builtin_popcount(pbs[b].mask);
             const int w = __builtin_popcount(pbs[b].mask | S);
             if (w <= std::max(std::max(kmax, k_here), have) && w - have <= best_g) {
@@ -215,11 +229,14 @@ int plan_blocks(const std::vector<Gate>& circuit, int n, int max_k, BlockPlan& o
             pbs[best].mask |= S;
             pbs[best].gates.push_back(gi);
         } else {
+            seq.push_back({0, (int)pbs.size()});
             pbs.push_back({S, {gi}});
         }
     }
     const uint32_t all = (1u << n) - 1;
-    for (const PB& pb : pbs) {
+    std::vector<std::pair<int, int>> where(pbs.size());  // pbs index -> order entry
+    for (size_t bi = 0; bi < pbs.size(); bi++) {
+        const PB& pb = pbs[bi];
         uint32_t qmask = pb.mask;
         for (int q = 0; q < n && __builtin_popcount(qmask) < 4; q++) qmask |= 1u << q;
         const int D = 1 << __builtin_popcount(qmask);
@@ -241,7 +258,7 @@ int plan_blocks(const std::vector<Gate>& circuit, int n, int max_k, BlockPlan& o
                 fb.ur[i] = U[i].real();
                 fb.ui[i] = U[i].imag();
             }
-            out.order.push_back({0, (int)out.blocks.size()});
+            where[bi] = {0, (int)out.blocks.size()};
             out.blocks.push_back(fb);
             out.members.push_back(pb.gates);
         } else {
@@ -254,11 +271,12 @@ int plan_blocks(const std::vector<Gate>& circuit, int n, int max_k, BlockPlan& o
                 fb.ur[i] = U[i].real();
                 fb.ui[i] = U[i].imag();
             }
-            out.order.push_back({2, (int)out.blocks5.size()});
+            where[bi] = {2, (int)out.blocks5.size()};
             out.blocks5.push_back(fb);
             out.members5.push_back(pb.gates);
         }
     }
+    for (const auto& e : seq) out.order.push_back(e.first == 3 ? e : where[e.second]);
     return DMX_OK;
 }
 
@@ -374,6 +392,13 @@ int build_fused_plan(const std::vector<Gate>& circuit, int n, int max_k, CostMod
     plan->blocks = std::move(bp.blocks);
     plan->blocks5 = std::move(bp.blocks5);
     for (const auto& o : bp.order) {
+        if (o.first == 3) {
+            const int first = (int)plan->prims.size();
+            lower_gate(circuit[o.second], plan->prims);  // the two orientation records
+            plan->steps.push_back({ST_CHANNEL, first, 2});
+            pi.channels++;
+            continue;
+        }
         if (o.first == 2) {
             plan->steps.push_back({ST_FUSED5, o.second, 0});
             pi.fused5_blocks++;

@@ -17,13 +17,16 @@ struct FusedPlan {
     virtual ~FusedPlan() = default;
     virtual int upload(hipStream_t stream) = 0;
     // Applies the whole circuit once on `stream`, timing launches into prof.
-    virtual int execute(double* re, double* im, int n, hipStream_t stream, LaunchProfiler& prof,
-                        uint32_t* launches, uint32_t* sweeps) = 0;
+    // transposed: rho is held conj-transposed (after a plain run), which
+    // selects the channel superoperators' records.
+    virtual int execute(double* re, double* im, int n, bool transposed, hipStream_t stream,
+                        LaunchProfiler& prof, uint32_t* launches, uint32_t* sweeps) = 0;
 };
 
 // Blocks of a circuit in execution order: order[i] = (0, index into blocks)
 // for a block of at most four qubits (padded to four), (2, index into
-// blocks5) for a five-qubit block.
+// blocks5) for a five-qubit block, (3, circuit index) for a channel, which no
+// gate crosses.
 struct BlockPlan {
     std::vector<FusedBlock> blocks;
     std::vector<std::vector<int>> members;  // circuit indices of each block's gates, in order
@@ -46,6 +49,7 @@ struct PlanInfo {
     uint32_t fused_blocks = 0;   // four-qubit blocks run fused
     uint32_t plain_blocks = 0;   // four-qubit blocks run as primitive sweeps on both indices
     uint32_t fused5_blocks = 0;  // five-qubit blocks (always fused)
+    uint32_t channels = 0;       // noise channels (one superoperator sweep each)
     uint32_t tile_hist[9] = {};  // fused blocks per tile width m (4..8)
     double predicted_us = 0.0;
     double greedy_us = 0.0;

@@ -144,7 +144,7 @@ const char* const kOpNames[OP_COUNT] = {
     "U3", "U2", "U1", "CX", "ID", "X", "Y", "Z", "H", "S",
     "SDG", "T", "TDG", "RX", "RY", "RZ", "CZ", "CY", "SWAP", "CH",
     "CCX", "CSWAP", "CRX", "CRY", "CRZ", "CU1", "CU3", "RXX", "RZZ", "RCCX",
-    "RC3X", "C3X", "C3SQRTX", "C4X", "R", "SRN", "W", "RYY", "MAT"};
+    "RC3X", "C3X", "C3SQRTX", "C4X", "R", "SRN", "W", "RYY", "MAT", "KRAUS"};
 
 int op_arity(int op)
 {
@@ -252,6 +252,17 @@ bool lower_gate(const Gate& g, std::vector<PrimOp>& out)
     case OP_RYY:
         E.rx(a, kPi / 2); E.rx(b, kPi / 2); E.cx(a, b); E.u1(b, theta); E.cx(a, b);
         E.rx(a, -kPi / 2); E.rx(b, -kPi / 2);
+        break;
+    case OP_KRAUS:
+        // two records: the superoperator for each orientation of the stored rho
+        for (int t = 0; t < 2; t++) {
+            PrimOp& p = E.push(PK_SUP, a, a);
+            p.pad = t;
+            for (int i = 0; i < 16; i++) {
+                p.c[2 * i] = g.mre[16 * t + i];
+                p.c[2 * i + 1] = g.mim[16 * t + i];
+            }
+        }
         break;
     case OP_MAT:
         if (g.k == 1) {

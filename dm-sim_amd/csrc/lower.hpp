@@ -15,7 +15,8 @@ enum ApiOp : int32_t {
     OP_SDG, OP_T, OP_TDG, OP_RX, OP_RY, OP_RZ, OP_CZ, OP_CY, OP_SWAP, OP_CH,
     OP_CCX, OP_CSWAP, OP_CRX, OP_CRY, OP_CRZ, OP_CU1, OP_CU3, OP_RXX, OP_RZZ, OP_RCCX,
     OP_RC3X, OP_C3X, OP_C3SQRTX, OP_C4X, OP_R, OP_SRN, OP_W, OP_RYY,
-    OP_MAT = 38,  // arbitrary matrix on k qubits (k in 1..5), see Gate::mat
+    OP_MAT = 38,    // arbitrary matrix on k qubits (k in 1..5), see Gate::mat
+    OP_KRAUS = 39,  // single-qubit channel sum_i K_i rho K_i^dagger, see Gate::mat
     OP_COUNT
 };
 
@@ -32,8 +33,9 @@ struct Gate {
     uint32_t qb[5] = {0, 0, 0, 0, 0};
     double theta = 0, phi = 0, lambda = 0;
     int32_t k = 0;              // OP_MAT only
-    std::vector<double> mre;    // OP_MAT only, row-major
-    std::vector<double> mim;
+    std::vector<double> mre;    // OP_MAT: the matrix, row-major; OP_KRAUS: the 4x4
+    std::vector<double> mim;    // superoperator on (row bit, column bit) of the stored
+                                // conj(rho), then its conj-transposed-orientation twin
     std::string dump() const;   // Gate::dump format (src/dmsim_amdgpu_omp.hpp:185-191)
 };
 

@@ -29,6 +29,9 @@ enum PrimKind : int32_t {
     PK_R,       // c[0] = phase
     PK_U1,      // c[0] = cos(lambda), c[1] = sin(lambda)
     PK_C2,      // general 4x4, c[0..31] = row-major complex (re, im) pairs
+    PK_SUP,     // channel superoperator: 4x4 on (row bit q0 + n, column bit q0), applied
+                // once (not per index side); pad = 0 for rho's stored orientation, 1 for
+                // the conj-transposed one
     PK_NUM
 };
 

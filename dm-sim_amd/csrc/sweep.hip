@@ -413,6 +413,17 @@ __global__ void __launch_bounds__(256) k_quad(double* re, double* im,
                      Range{gtid, gthreads, 0});
 }
 
+// One channel superoperator (PK_SUP): the 4x4 acts on the entry quad of row
+// bit q0 + n and column bit q0, in the quad order of C2.
+__global__ void __launch_bounds__(256) k_super(double* re, double* im,
+                                               const PrimOp* __restrict__ op, int n)
+{
+    const uint64_t N = 1ull << (2 * n);
+    const uint64_t gtid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const uint64_t gthreads = (uint64_t)gridDim.x * blockDim.x;
+    sweep_quad<false>(re, im, op->c, op->q0 + n, op->q0, N, Range{gtid, gthreads, 0});
+}
+
 // In-place conjugate transpose with 32x32 LDS tiles; block (tj, ti) swaps tile
 // (ti, tj) with tile (tj, ti); blocks below the diagonal exit.
 __global__ void __launch_bounds__(256) k_conj_transpose(double* re, double* im, int n)
@@ -546,6 +557,12 @@ hipError_t launch_plain_one(double* re, double* im, const PrimOp* d_op, const Pr
 {
     const uint64_t N = 1ull << (2 * n);
     const int shift = slow ? n : 0;
+    if (host_op.kind == PK_SUP) {
+        const uint64_t need = (N / 4 + 255) / 256;
+        const unsigned blocks = (unsigned)(need < (uint64_t)max_blocks ? need : max_blocks);
+        k_super<<<blocks, 256, 0, stream>>>(re, im, d_op, n);
+        return hipGetLastError();
+    }
     if (host_op.kind == PK_C2) {
         const uint64_t need = (N / 4 + 255) / 256;
         const unsigned blocks = (unsigned)(need < (uint64_t)max_blocks ? need : max_blocks);

@@ -24,7 +24,7 @@ hipError_t launch_plain_one(double* re, double* im, const PrimOp* d_op, const Pr
 // both planes) of one primitive sweep over 4^n entries.
 inline int prim_class(const PrimOp& p)
 {
-    if (p.kind == PK_C2) return KC_QUAD;
+    if (p.kind == PK_C2 || p.kind == PK_SUP) return KC_QUAD;
     if (p.kind == PK_X && p.q1 >= 0) return KC_CX;
     return pos1_only(p.kind) ? KC_PAIR_POS1 : KC_PAIR_DENSE;
 }

@@ -8,6 +8,7 @@ not been built: there is no fallback path.
 from __future__ import annotations
 
 import ctypes as C
+import math
 import os
 
 import numpy as np
@@ -22,7 +23,8 @@ COST_FEATURES, COST_PARAMS = 12, 5057
 
 # every symbol include/dmsim_amd.h declares
 EXPORTS = [
-    "dmx_create", "dmx_destroy", "dmx_append", "dmx_append_matrix", "dmx_upload", "dmx_run",
+    "dmx_create", "dmx_destroy", "dmx_append", "dmx_append_matrix", "dmx_append_kraus",
+    "dmx_upload", "dmx_run",
     "dmx_clear_circuit", "dmx_reset", "dmx_reset_dm", "dmx_measure", "dmx_get_dm",
     "dmx_get_entries",
     "dmx_get_diag", "dmx_expect_z", "dmx_info", "dmx_dump", "dmx_set_method",
@@ -61,6 +63,7 @@ def load():
     lib.dmx_append.argtypes = [vp, C.c_int32, P(C.c_uint32), C.c_double, C.c_double, C.c_double]
     lib.dmx_append_matrix.argtypes = [vp, C.c_uint32, P(C.c_uint32), P(C.c_double),
                                       P(C.c_double)]
+    lib.dmx_append_kraus.argtypes = [vp, C.c_uint32, C.c_uint32, P(C.c_double), P(C.c_double)]
     for name in ("dmx_upload", "dmx_run", "dmx_clear_circuit", "dmx_reset", "dmx_reset_dm"):
         getattr(lib, name).argtypes = [vp]
     lib.dmx_measure.argtypes = [vp, C.c_uint32, C.c_uint32, P(C.c_uint32), P(C.c_double)]
@@ -171,6 +174,14 @@ class Sim:
         q = np.asarray(qubits, dtype=np.uint32)
         _check(load().dmx_append_matrix(self._h, len(q), _ptr(q, C.c_uint32),
                                         _ptr(re, C.c_double), _ptr(im, C.c_double)))
+
+    def append_kraus(self, qubit: int, ops):
+        """Noise channel rho -> sum_i K_i rho K_i^dagger on one qubit (2x2 operators)."""
+        ks = np.ascontiguousarray(np.asarray(ops, dtype=complex).reshape(-1, 2, 2))
+        re = np.ascontiguousarray(ks.real.ravel())
+        im = np.ascontiguousarray(ks.imag.ravel())
+        _check(load().dmx_append_kraus(self._h, qubit, len(ks), _ptr(re, C.c_double),
+                                       _ptr(im, C.c_double)))
 
     def append_table(self, table):
         """Append every gate of a workloads.GateTable."""
@@ -307,6 +318,27 @@ class Sim:
         _check(load().dmx_time_launch(self._h, kind, m, qmask, target, control, slow, reps,
                                       C.byref(us)))
         return us.value
+
+
+def depolarizing(p):
+    """Kraus operators of the single-qubit depolarizing channel with error
+    probability p: (1 - p) rho + p/3 (X rho X + Y rho Y + Z rho Z)."""
+    i2 = np.eye(2)
+    x = np.array([[0, 1], [1, 0]])
+    y = np.array([[0, -1j], [1j, 0]])
+    z = np.diag([1, -1])
+    return [math.sqrt(1 - p) * i2] + [math.sqrt(p / 3) * m for m in (x, y, z)]
+
+
+def amplitude_damping(gamma):
+    """Kraus operators of amplitude damping with decay probability gamma."""
+    return [np.array([[1, 0], [0, math.sqrt(1 - gamma)]]),
+            np.array([[0, math.sqrt(gamma)], [0, 0]])]
+
+
+def phase_damping(lam):
+    """Kraus operators of phase damping with parameter lam."""
+    return [np.array([[1, 0], [0, math.sqrt(1 - lam)]]), np.array([[0, 0], [0, math.sqrt(lam)]])]
 
 
 def qasm_parse(source: str):

@@ -73,6 +73,16 @@ int dmx_append(dmx_sim* sim, int32_t op, const uint32_t qb[5], double theta, dou
 int dmx_append_matrix(dmx_sim* sim, uint32_t k, const uint32_t* qubits, const double* mat_re,
                       const double* mat_im);
 
+/* Extension with no counterpart in the reference (the DM-Sim paper's Kraus
+ * form E(rho) = sum_i K_i rho K_i^dagger, which the v2.5 code does not
+ * implement): a single-qubit noise channel given by n_ops 2x2 Kraus operators
+ * (k_re/k_im[4*i..4*i+3], row-major).  It acts once on both indices of rho as
+ * a 4x4 superoperator sweep; in the fused method it is a barrier no gate is
+ * moved across.  Trace preservation (sum_i K_i^dagger K_i = I) is the
+ * caller's to ensure. */
+int dmx_append_kraus(dmx_sim* sim, uint32_t qubit, uint32_t n_ops, const double* k_re,
+                     const double* k_im);
+
 /* Simulation::upload(), src/dmsim_amdgpu_omp.hpp:360-392: one packed copy of
  * the lowered gate table to the device.  A second upload() without
  * clear_circuit() in between is DMX_ESTATE (the reference asserts). */

@@ -1,8 +1,11 @@
 // The reference's 10-qubit adder example written against the C++ face of
 // this engine (include/dmsim_amd.hpp): cin = 0, a = 0001, b = 1111; every shot
 // reads 1000000010 (= 514).  Exits non-zero if the state or the shots differ.
+// Optional argument: execution method (0 persistent plain, the default;
+// 1 plain, one launch per primitive; 2 fused).
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 
 #include "dmsim_amd.hpp"
 
@@ -22,9 +25,10 @@ static void unmaj(Simulation& sim, IdxType a, IdxType b, IdxType c)
     sim.append(Simulation::CX(a, b));
 }
 
-int main()
+int main(int argc, char** argv)
 {
     Simulation sim(10, 1);
+    if (argc > 1) detail::check(dmx_set_method(sim.handle(), std::atoi(argv[1])));
     for (IdxType q : {1u, 5u, 6u, 7u, 8u}) {
         Gate* g = Simulation::X(q);
         sim.append(g);
