@@ -93,22 +93,24 @@ def test_pybind_rejects_multi_gpu_without_touching_gpu():
         m.Simulation(4, 2)
 
 
-def build_cpp_facade_example(out_dir):
-    """Compile tests/cpp/adder_facade.cpp against include/dmsim_amd.hpp and the
-    library; returns the executable's path."""
+def build_cpp_facade_example(out_dir, name="adder_facade"):
+    """Compile tests/cpp/<name>.cpp against include/ and the library; returns
+    the executable's path."""
     import subprocess
-    exe = os.path.join(str(out_dir), "adder_facade")
+    exe = os.path.join(str(out_dir), name)
     pkg = os.path.join(ROOT, "dm-sim_amd")
     cmd = ["g++", "-std=c++17", "-O2", "-Wall", "-I", os.path.join(ROOT, "include"),
-           os.path.join(ROOT, "tests", "cpp", "adder_facade.cpp"), "-L", pkg, "-ldmsim_amd",
+           os.path.join(ROOT, "tests", "cpp", name + ".cpp"), "-L", pkg, "-ldmsim_amd",
            f"-Wl,-rpath,{pkg}", "-o", exe]
     r = subprocess.run(cmd, capture_output=True, text=True)
     assert r.returncode == 0, r.stderr
     return exe
 
 
-def test_cpp_facade_builds(tmp_path):
+@pytest.mark.parametrize("name", ["adder_facade", "xacc_backend"])
+def test_cpp_faces_build(tmp_path, name):
     """The reference-style C++ program (namespace DMSim, Simulation::CX(...),
-    sim.sim(), measure, print_measurement, dm_real_res) compiles and links
-    against the header-only face of the C ABI."""
-    assert os.path.exists(build_cpp_facade_example(tmp_path))
+    sim.sim(), measure, print_measurement, dm_real_res) and the XACC backend
+    interface (namespace DmSim, DmSimBackend) compile and link against the
+    header-only faces of the C ABI."""
+    assert os.path.exists(build_cpp_facade_example(tmp_path, name))

@@ -260,3 +260,15 @@ def test_cpp_facade_adder(tmp_path):
     r = subprocess.run([exe, "1"], capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stdout + r.stderr
     assert r.stdout.count("1000000010") == 5
+
+
+def test_xacc_backend_interface(tmp_path):
+    """The reference's XACC deuteron check, CH semantics and the ops its
+    runners reject, through the DmSimBackend interface (tests/cpp/xacc_backend.cpp)."""
+    import subprocess
+    from test_capi_host import build_cpp_facade_example
+    exe = build_cpp_facade_example(tmp_path, "xacc_backend")
+    print("xacc backend: running", flush=True)
+    r = subprocess.run([exe], capture_output=True, text=True, timeout=120)
+    print(r.stdout, flush=True)
+    assert r.returncode == 0, r.stdout + r.stderr
