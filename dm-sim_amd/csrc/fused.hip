@@ -253,8 +253,11 @@ hipError_t launch_fused4_batch(const BatchEntry* d_entries, int count, int n, in
     if (n < 4 || m < 4 || m > 8 || m > n || count < 1 || count > 65535)
         return hipErrorInvalidValue;
     const uint64_t n_st = supertiles(n, m);
-    const dim3 grid((unsigned)(n_st < (uint64_t)max_blocks ? n_st : (uint64_t)max_blocks),
-                    (unsigned)count);
+    // about max_blocks workgroups in all, each looping over super-tiles of its
+    // circuit (one workgroup per super-tile re-runs the set-up for every tile)
+    uint64_t per = ((uint64_t)max_blocks + count - 1) / count;
+    if (per < 1) per = 1;
+    const dim3 grid((unsigned)(n_st < per ? n_st : per), (unsigned)count);
     switch (m) {
     case 4: k_fused4_batch<4><<<grid, 256, 0, stream>>>(d_entries, n); break;
     case 5: k_fused4_batch<5><<<grid, 256, 0, stream>>>(d_entries, n); break;

@@ -62,7 +62,8 @@ hipError_t launch_fused5(double* re, double* im, const FusedBlock5* d_blk, int n
 hipError_t launch_fused4(double* re, double* im, const FusedBlock* d_blk, int n, int m,
                          int max_blocks, hipStream_t stream);
 
-// The same for `count` circuits of n qubits each (one launch).
+// The same for `count` circuits of n qubits each (one launch of about
+// max_blocks workgroups in all).
 hipError_t launch_fused4_batch(const BatchEntry* d_entries, int count, int n, int m,
                                int max_blocks, hipStream_t stream);
 
