@@ -7,8 +7,9 @@
 
 namespace dmx {
 
-// Blocks of 256 threads for the persistent loop: co-resident maximum on this
-// device, capped at one wave per 256 entries of a plane.
+// Blocks of 256 threads for the persistent loop: 4^n / 8192 clamped to
+// 16..256 (n <= 12) or 16..512, at most one wave per 256 entries of a plane
+// and at most the co-resident maximum on this device.
 hipError_t plain_persistent_grid(int n, int* blocks);
 
 // Pair primitives [first, last) of one pass (slow: row index) in one
