@@ -105,3 +105,46 @@ def test_expressions(dm):
 def test_errors_name_the_line(dm, src, msg):
     with pytest.raises(ValueError, match=msg):
         dm.qasm_parse(src)
+
+
+def test_adder_file_is_the_reference_example():
+    """Pins both readers to an independent source: the reference's own Python
+    example (example/adder_n10_omp.py:38-82, restated in workloads.adder_n10)
+    builds the Cuccaro adder that benchmark/adder_n9.qasm describes, gate for
+    gate (cin -> 0, a -> 1..4, b -> 5..8, cout -> 9)."""
+    import workloads as W
+    first, second = W.adder_n10()
+    want = []
+    for t in (first, second):
+        ops, qb, _, _ = t.arrays()
+        want += [(int(o), tuple(int(x) for x in q[:W.ARITY[W.OP_NAMES[o]]])) for o, q in zip(ops, qb)]
+    src = open(os.path.join(HERE, "golden", "qasm", "adder_n9.qasm")).read()
+    nq, gates = qo.translate(src)
+    assert nq == 10 and [(g[0], g[1]) for g in gates] == want
+    dmx = pytest.importorskip("dmsim_amd")
+    _, cgates = table(dmx, src)
+    assert [(g[0], g[1][:len(w[1])]) for g, w in zip(cgates, want)] == want
+
+
+def test_oracle_broadcast_and_slots_known_answer():
+    """The restated translator on a program with a known gate table: register
+    offsets in declaration order, whole-register broadcast, parameter slots of
+    the factories (RZ -> phi, CU1 -> lambda, U2 -> phi and lambda)."""
+    src = """OPENQASM 2.0;
+include "qelib1.inc";
+qreg a[2];
+qreg b[3];
+h b;
+cx a[1],b[2];
+rz(pi/2) a[0];
+cu1(-pi/4) b[0],a[1];
+u2(0.25,0.5) b[1];
+"""
+    nq, gates = qo.translate(src)
+    OP = qo.OP
+    assert nq == 5
+    assert gates == [(OP["H"], (2,), (0.0, 0.0, 0.0)), (OP["H"], (3,), (0.0, 0.0, 0.0)),
+                     (OP["H"], (4,), (0.0, 0.0, 0.0)), (OP["CX"], (1, 4), (0.0, 0.0, 0.0)),
+                     (OP["RZ"], (0,), (0.0, math.pi / 2, 0.0)),
+                     (OP["CU1"], (2, 1), (0.0, 0.0, -math.pi / 4)),
+                     (OP["U2"], (3,), (0.0, 0.25, 0.5))]
