@@ -2,10 +2,10 @@
 //
 // Each circuit is planned into <= 4-qubit blocks by the fused planner.  Run
 // order: reset every rho in the pool; for block step s = 0, 1, ..., one
-// launch per qubit count n covering every circuit of that width that still
-// has a block s (grid.y = circuit); then one workgroup per circuit sums its
-// Pauli terms.  The sequence is captured once into a hipGraph and replayed by
-// each run.
+// launch per (qubit count, tile width) covering every circuit of that width
+// that still has a block s (grid.y = circuit); then one workgroup per circuit
+// sums its Pauli terms.  The sequence is captured once into a hipGraph and
+// replayed by each run.
 #include "batch.hpp"
 
 #include <algorithm>

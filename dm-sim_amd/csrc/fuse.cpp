@@ -34,10 +34,11 @@ namespace {
 
 using cd = std::complex<double>;
 
-// A run is a sequence of steps: a fused block (one sweep), or a plain segment
-// of primitive sweeps (a gate wider than the block width, or a block the cost
-// model runs unfused), applied to the fast index and then to the slow index
-// (valid for complex-linear gates, whose row and column actions commute).
+// A run is a sequence of steps: a fused four- or five-qubit block (one sweep
+// each); a plain segment, the primitive sweeps of a block the cost model runs
+// unfused, applied to the fast index and then to the slow index (valid for
+// complex-linear gates, whose row and column actions commute); or a channel
+// (one superoperator sweep, in the record for rho's current orientation).
 enum StepType { ST_FUSED4 = 0, ST_PLAIN = 1, ST_FUSED5 = 2, ST_CHANNEL = 3 };
 
 struct Step {

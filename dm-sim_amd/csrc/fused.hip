@@ -9,11 +9,12 @@
 // inside the super-tile, so every global access is a contiguous run of at
 // least 16 doubles.  Threads move column pairs, so column bit 0 must be in
 // the super-tile (m > 4, or qubit 0 in Q).
-// Each of the 16 tiles of a super-tile is transformed by one wave:
+// The 2^(m-4) tiles of a super-tile are dealt round-robin to the four waves;
+// each tile is transformed by one wave:
 //   X  = T U^T      (12 MFMAs: complex product from three real products)
 //   T' = conj(U) X  (12 MFMAs)
-// Two workgroups per CU (64 KiB of LDS each) overlap one another's load,
-// transform and store phases.
+// Two workgroups per CU (64 KiB of LDS each at m = 8) overlap one another's
+// load, transform and store phases.
 // MFMA lane layout on gfx950 (measured): A[i][k] in lane i+16k, B[k][j] in
 // lane j+16k, D[4r+g][j] in lane j+16g register r -- so register c of X is
 // exactly the B operand of the second product's K-chunk c, and the block
