@@ -114,6 +114,9 @@ const std::map<std::string, Builtin>& builtins()
     return t;
 }
 
+// Expanded gate count limit (nested user gates can multiply a short text).
+constexpr size_t kMaxGates = 1u << 22;
+
 struct GateDef {
     std::vector<std::string> params;
     std::vector<std::string> qargs;
@@ -288,6 +291,9 @@ struct Parser {
         g.theta = slot[0];
         g.phi = slot[1];
         g.lambda = slot[2];
+        if (prog->gates.size() >= kMaxGates)
+            throw QasmError(line, "the program expands to more than " +
+                                      std::to_string(kMaxGates) + " gates");
         prog->gates.push_back(std::move(g));
     }
 

@@ -148,3 +148,14 @@ u2(0.25,0.5) b[1];
                      (OP["RZ"], (0,), (0.0, math.pi / 2, 0.0)),
                      (OP["CU1"], (2, 1), (0.0, 0.0, -math.pi / 4)),
                      (OP["U2"], (3,), (0.0, 0.25, 0.5))]
+
+
+def test_expansion_limit(dm):
+    """Nested definitions that double at every level stop with an error
+    instead of exhausting memory (2^30 gates here)."""
+    lines = ["qreg q[1];", "gate g0 a { x a; }"]
+    for k in range(1, 31):
+        lines.append(f"gate g{k} a {{ g{k - 1} a; g{k - 1} a; }}")
+    lines.append("g30 q[0];")
+    with pytest.raises(ValueError, match="expands to more than"):
+        dm.qasm_parse("\n".join(lines))
