@@ -1,8 +1,8 @@
 """Timing-data generator and trainer for the fused planner's cost model.
 
 For each qubit count n it times, on this GPU, the launch configurations the
-planner chooses between (``dmx_time_launch``): the dense 4-qubit block sweep
-at every tile width for a spread of qubit sets, and the primitive sweeps of
+planner chooses between (``dmx_time_launch``): the dense 4- and 5-qubit block
+sweeps at every tile width for a spread of qubit sets, and the primitive sweeps of
 every kind on every target bit, on both index sides.  Targets are
 ln(microseconds).  It then fits the model (``dmx_cost_train``) and writes its
 parameters as an .npz that ``CostModel.load`` reads.
@@ -36,6 +36,14 @@ def launch_configs(n, rng, fused_sets=10):
         for qmask in sorted(sets):
             for m in range(4 if qmask & 1 else 5, min(8, n) + 1):  # m = 4 needs qubit 0
                 out.append((dm.COST_FUSED, m, qmask, 0, -1, 0))
+    if n >= 5:
+        sets = {0x1F, ((1 << 5) - 1) << (n - 5), 0b111 | (0b11 << (n - 2))}
+        while len(sets) < min(fused_sets, math.comb(n, 5)):
+            q = rng.choice(n, 5, replace=False)
+            sets.add(int(sum(1 << int(x) for x in q)))
+        for qmask in sorted(sets):
+            for m in range(5 if qmask & 1 else 6, min(8, n) + 1):  # m = 5 needs qubit 0
+                out.append((dm.COST_FUSED5, m, qmask, 0, -1, 0))
     for slow in (0, 1):
         for t in range(n):
             out.append((dm.COST_DENSE, 0, 0, t, -1, slow))
@@ -51,6 +59,8 @@ def launch_configs(n, rng, fused_sets=10):
 def est_us(n, kind):
     full = 32.0 * 4 ** n
     by = full / 2 if kind in (dm.COST_DIAG, dm.COST_CX) else full
+    if kind == dm.COST_FUSED5:
+        by *= 2  # MFMA-heavier than one streaming pass
     return 4.0 + by / 4.0e6
 
 

@@ -401,7 +401,7 @@ int dmx_cost_destroy(dmx_cost* model)
 int dmx_cost_features(uint32_t n_qubits, int32_t kind, int32_t m, uint32_t qmask, int32_t target,
                       int32_t control, int32_t slow, float* out)
 {
-    if (!out || n_qubits < 1 || n_qubits > 16 || kind < 0 || kind > DMX_COST_QUAD) {
+    if (!out || n_qubits < 1 || n_qubits > 16 || kind < 0 || kind > DMX_COST_FUSED5) {
         dmx::set_error("cost features need an output, 1 <= n <= 16 and a known kind");
         return DMX_EINVAL;
     }
@@ -411,6 +411,13 @@ int dmx_cost_features(uint32_t n_qubits, int32_t kind, int32_t m, uint32_t qmask
             !dmx::tile_bits_ok(n, m, qmask)) {
             dmx::set_error("fused features need n >= 4, four qubits < n and 4 <= m <= min(8, n) "
                            "(m = 4 only with qubit 0 in the block)");
+            return DMX_EINVAL;
+        }
+    } else if (kind == DMX_COST_FUSED5) {
+        if (n < 5 || __builtin_popcount(qmask) != 5 || (qmask >> n) != 0 ||
+            !dmx::tile_bits5_ok(n, m, qmask)) {
+            dmx::set_error("five-qubit features need n >= 5, five qubits < n and "
+                           "5 <= m <= min(8, n) (m = 5 only with qubit 0 in the block)");
             return DMX_EINVAL;
         }
     } else {

@@ -41,6 +41,7 @@ Engine::~Engine()
     if (d_diag_) (void)hipFree(d_diag_);
     if (d_ops_) (void)hipFree(d_ops_);
     if (d_probe_blk_) (void)hipFree(d_probe_blk_);
+    if (d_probe_blk5_) (void)hipFree(d_probe_blk5_);
     if (d_probe_op_) (void)hipFree(d_probe_op_);
     if (ev0_) (void)hipEventDestroy(ev0_);
     if (ev1_) (void)hipEventDestroy(ev1_);
@@ -429,6 +430,13 @@ int Engine::time_launch(int kind, int m, uint32_t qmask, int target, int control
                       "(m = 4 only with qubit 0 in the block)");
             return DMX_EINVAL;
         }
+    } else if (kind == CK_FUSED5) {
+        if (n_ < 5 || __builtin_popcount(qmask) != 5 || (qmask >> n_) != 0 ||
+            !tile_bits5_ok(n_, m, qmask)) {
+            set_error("five-qubit timing needs n >= 5, five qubits < n and 5 <= m <= min(8, n) "
+                      "(m = 5 only with qubit 0 in the block)");
+            return DMX_EINVAL;
+        }
     } else if (kind >= CK_DENSE && kind <= CK_QUAD) {
         const bool two = kind == CK_CX || kind == CK_QUAD;
         if (target < 0 || target >= n_ || (two && (control < 0 || control >= n_ || control == target))) {
@@ -461,8 +469,21 @@ int Engine::time_launch(int kind, int m, uint32_t qmask, int target, int control
         return DMX_EINVAL;
     }
     if (!d_probe_blk_) HIPTRY(hipMalloc(&d_probe_blk_, sizeof(FusedBlock)));
+    if (!d_probe_blk5_) HIPTRY(hipMalloc(&d_probe_blk5_, sizeof(FusedBlock5)));
     if (!d_probe_op_) HIPTRY(hipMalloc(&d_probe_op_, sizeof(PrimOp)));
-    if (kind == CK_FUSED) {  // 16 x 16 DFT / 4
+    if (kind == CK_FUSED5) {  // 32 x 32 DFT / sqrt(32)
+        std::vector<FusedBlock5> fb(1);
+        std::memset(fb.data(), 0, sizeof(FusedBlock5));
+        fb[0].qmask = qmask;
+        fb[0].m = (uint32_t)m;
+        const double s = 1.0 / std::sqrt(32.0);
+        for (int r = 0; r < 32; r++)
+            for (int c = 0; c < 32; c++) {
+                fb[0].ur[32 * r + c] = s * std::cos(2 * pi * r * c / 32);
+                fb[0].ui[32 * r + c] = s * std::sin(2 * pi * r * c / 32);
+            }
+        HIPTRY(hipMemcpy(d_probe_blk5_, fb.data(), sizeof(FusedBlock5), hipMemcpyHostToDevice));
+    } else if (kind == CK_FUSED) {  // 16 x 16 DFT / 4
         FusedBlock fb;
         std::memset(&fb, 0, sizeof(fb));
         fb.qmask = qmask;
@@ -477,6 +498,8 @@ int Engine::time_launch(int kind, int m, uint32_t qmask, int target, int control
         HIPTRY(hipMemcpy(d_probe_op_, &op, sizeof(op), hipMemcpyHostToDevice));
     }
     auto launch = [&]() {
+        if (kind == CK_FUSED5)
+            return launch_fused5(d_re_, d_im_, d_probe_blk5_, n_, m, kFusedGridCap, stream_);
         return kind == CK_FUSED
                    ? launch_fused4(d_re_, d_im_, d_probe_blk_, n_, m, kFusedGridCap, stream_)
                    : launch_plain_one(d_re_, d_im_, d_probe_op_, op, n_, slow != 0, kSweepGridCap,

@@ -89,6 +89,7 @@ private:
     int tile_bits_ = 0;
     PlanInfo plan_info_;
     FusedBlock* d_probe_blk_ = nullptr;  // time_launch's operator records
+    FusedBlock5* d_probe_blk5_ = nullptr;
     PrimOp* d_probe_op_ = nullptr;
     LaunchProfiler prof_;
 

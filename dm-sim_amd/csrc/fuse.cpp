@@ -367,6 +367,70 @@ int choose_executions(const std::vector<Gate>& circuit, int n, BlockPlan& bp, Co
     return DMX_OK;
 }
 
+// Learned choice per five-qubit block: one k_fused5 sweep at tile width m
+// (5..min(8, n)), or its member gates re-planned into four-qubit blocks that
+// are priced and chosen as above.  Sets blocks5[b].m; sub[b].used marks the
+// blocks that run split, with their four-qubit plan.
+struct SubPlan {
+    bool used = false;
+    BlockPlan bp;
+    std::vector<std::vector<PrimOp>> plain;
+};
+
+int choose_fused5(const std::vector<Gate>& circuit, int n, BlockPlan& bp, CostModel& model,
+                  std::vector<SubPlan>& sub, PlanInfo& info)
+{
+    const int nb = (int)bp.blocks5.size();
+    sub.assign(nb, SubPlan{});
+    if (nb == 0) return DMX_OK;
+    const int m_hi = std::min(8, n);
+    const int n_m = m_hi - 5 + 1;
+    std::vector<std::vector<float>> feats;
+    float f[kMlpFeatures];
+    for (int b = 0; b < nb; b++)
+        for (int m = 5; m <= m_hi; m++) {
+            const int mm = tile_bits5_ok(n, m, bp.blocks5[b].qmask) ? m : m_hi;
+            cost_features(n, CK_FUSED5, mm, bp.blocks5[b].qmask, 0, -1, 0, f);
+            feats.emplace_back(f, f + kMlpFeatures);
+        }
+    std::vector<float> us;
+    int rc = model.price(feats, us);
+    if (rc) return rc;
+    for (int b = 0; b < nb; b++) {
+        const float* t = us.data() + (size_t)b * n_m;
+        const int md = (int)bp.blocks5[b].m;  // the default width plan_blocks chose
+        int best_m = md;
+        double best = t[md - 5];
+        for (int i = 0; i < n_m; i++)
+            if (tile_bits5_ok(n, 5 + i, bp.blocks5[b].qmask) && t[i] < kMargin * t[md - 5] &&
+                t[i] < best) {
+                best_m = 5 + i;
+                best = t[i];
+            }
+        info.greedy_us += t[md - 5];
+        bp.blocks5[b].m = (uint32_t)best_m;
+        std::vector<Gate> gates;
+        for (int gi : bp.members5[b]) gates.push_back(circuit[gi]);
+        SubPlan sp;
+        bool fusable = false;
+        rc = plan_blocks(gates, n, 4, sp.bp, fusable);
+        if (rc) return rc;
+        if (fusable && sp.bp.blocks5.empty()) {  // a five-qubit gate cannot be split
+            PlanInfo si;
+            rc = choose_executions(gates, n, sp.bp, model, sp.plain, si);
+            if (rc) return rc;
+            if (si.predicted_us < kMargin * best) {
+                best = si.predicted_us;
+                info.plain_blocks += si.plain_blocks;
+                sp.used = true;
+                sub[b] = std::move(sp);
+            }
+        }
+        info.predicted_us += best;
+    }
+    return DMX_OK;
+}
+
 }  // namespace
 
 int build_fused_plan(const std::vector<Gate>& circuit, int n, int max_k, CostModel* model,
@@ -379,6 +443,7 @@ int build_fused_plan(const std::vector<Gate>& circuit, int n, int max_k, CostMod
     int rc = plan_blocks(circuit, n, max_k, bp, fusable);
     if (rc || !fusable) return rc;
     std::vector<std::vector<PrimOp>> plain(bp.blocks.size());
+    std::vector<SubPlan> sub(bp.blocks5.size());
     if (tile_bits > 0) {
         for (FusedBlock& fb : bp.blocks)
             if (tile_bits_ok(n, tile_bits, fb.qmask)) fb.m = (uint32_t)tile_bits;
@@ -386,6 +451,8 @@ int build_fused_plan(const std::vector<Gate>& circuit, int n, int max_k, CostMod
             if (tile_bits5_ok(n, tile_bits, fb.qmask)) fb.m = (uint32_t)tile_bits;
     } else if (model) {
         rc = choose_executions(circuit, n, bp, *model, plain, pi);
+        if (rc) return rc;
+        rc = choose_fused5(circuit, n, bp, *model, sub, pi);
         if (rc) return rc;
     }
     Plan* plan = new Plan();
@@ -400,7 +467,22 @@ int build_fused_plan(const std::vector<Gate>& circuit, int n, int max_k, CostMod
             pi.channels++;
             continue;
         }
-        if (o.first == 2) {
+        if (o.first == 2 && sub[o.second].used) {  // the block's four-qubit plan
+            SubPlan& sp = sub[o.second];
+            for (const auto& so : sp.bp.order) {
+                const int j = so.second;
+                if (sp.plain[j].empty()) {
+                    plan->steps.push_back({ST_FUSED4, (int)plan->blocks.size(), 0});
+                    plan->blocks.push_back(sp.bp.blocks[j]);
+                    pi.fused_blocks++;
+                    pi.tile_hist[sp.bp.blocks[j].m]++;
+                } else {
+                    const int first = (int)plan->prims.size();
+                    plan->prims.insert(plan->prims.end(), sp.plain[j].begin(), sp.plain[j].end());
+                    plan->steps.push_back({ST_PLAIN, first, (int)plan->prims.size() - first});
+                }
+            }
+        } else if (o.first == 2) {
             plan->steps.push_back({ST_FUSED5, o.second, 0});
             pi.fused5_blocks++;
         } else if (plain[o.second].empty()) {

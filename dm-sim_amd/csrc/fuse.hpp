@@ -48,7 +48,8 @@ class CostModel;  // mlp.hpp
 struct PlanInfo {
     uint32_t fused_blocks = 0;   // four-qubit blocks run fused
     uint32_t plain_blocks = 0;   // four-qubit blocks run as primitive sweeps on both indices
-    uint32_t fused5_blocks = 0;  // five-qubit blocks (always fused)
+    uint32_t fused5_blocks = 0;  // five-qubit blocks run fused (a split one counts as its
+                                 // four-qubit sub-blocks instead)
     uint32_t channels = 0;       // noise channels (one superoperator sweep each)
     uint32_t tile_hist[9] = {};  // fused blocks per tile width m (4..8)
     double predicted_us = 0.0;
@@ -56,9 +57,10 @@ struct PlanInfo {
 };
 
 // Plan of a circuit for the fused executor.  With a cost model, each block is
-// priced as a fused launch at every tile width and as its primitive sweeps,
-// and the cheapest is taken; without one, every block is fused at the
-// default tile width (the greedy plan).
+// priced as a fused launch at every tile width and as its primitive sweeps
+// (a five-qubit block: as its member gates re-planned into four-qubit
+// blocks), and the cheapest is taken; without one, every block is fused at
+// the default tile width (the greedy plan).
 // tile_bits > 0 fixes every fused block's tile width where that width is
 // admissible for the block (tile_bits_ok), and takes precedence over the model.
 int build_fused_plan(const std::vector<Gate>& circuit, int n, int max_k, CostModel* model,

@@ -18,8 +18,8 @@ LIB_PATH = os.path.join(HERE, "libdmsim_amd.so")
 
 DMX_OK, DMX_EINVAL, DMX_ESTATE, DMX_EHIP, DMX_ENOMEM = 0, -1, -2, -3, -4
 METHOD_PLAIN, METHOD_PLAIN_LAUNCH, METHOD_FUSED = 0, 1, 2
-COST_FUSED, COST_DENSE, COST_DIAG, COST_CX, COST_QUAD = 0, 1, 2, 3, 4
-COST_FEATURES, COST_PARAMS = 12, 5057
+COST_FUSED, COST_DENSE, COST_DIAG, COST_CX, COST_QUAD, COST_FUSED5 = 0, 1, 2, 3, 4, 5
+COST_FEATURES, COST_PARAMS = 13, 5121
 
 # every symbol include/dmsim_amd.h declares
 EXPORTS = [

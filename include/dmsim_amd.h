@@ -189,8 +189,8 @@ int dmx_append_qasm(dmx_sim* sim, const char* source, uint32_t* n_gates);
  * kernel launch.  The fused planner prices every block as a fused sweep at
  * each tile width and as its gates' primitive sweeps, and keeps the cheapest;
  * without a model it fuses every block at the default width (greedy). ---- */
-#define DMX_COST_FEATURES 12
-#define DMX_COST_PARAMS 5057
+#define DMX_COST_FEATURES 13
+#define DMX_COST_PARAMS 5121
 
 /* Launch kinds the model prices. */
 enum dmx_cost_kind {
@@ -198,7 +198,8 @@ enum dmx_cost_kind {
     DMX_COST_DENSE = 1, /* 2x2 primitive rewriting both entries, one index */
     DMX_COST_DIAG = 2,  /* diagonal primitive (only the bit-set entry changes) */
     DMX_COST_CX = 3,    /* controlled swap */
-    DMX_COST_QUAD = 4   /* 4x4 primitive */
+    DMX_COST_QUAD = 4,  /* 4x4 primitive */
+    DMX_COST_FUSED5 = 5 /* dense 5-qubit block, both indices, one sweep */
 };
 
 typedef struct dmx_cost dmx_cost;
@@ -207,7 +208,8 @@ int dmx_cost_create(uint32_t seed, dmx_cost** out);
 int dmx_cost_destroy(dmx_cost* model);
 
 /* Feature vector (DMX_COST_FEATURES floats) of one launch on n qubits.  fused:
- * m = super-tile column bits (4..min(8, n)), qmask = the block's four qubits;
+ * m = super-tile column bits (4..min(8, n); five-qubit blocks 5..min(8, n)),
+ * qmask = the block's four (five) qubits;
  * sweeps: target (4x4: high qubit), control (cx control, 4x4 low qubit, else
  * -1), slow = 1 for the slow (row) index.  Host only. */
 int dmx_cost_features(uint32_t n_qubits, int32_t kind, int32_t m, uint32_t qmask, int32_t target,
@@ -245,11 +247,12 @@ int dmx_set_cost_model(dmx_sim* sim, dmx_cost* model);
 int dmx_set_tile_bits(dmx_sim* sim, uint32_t m);
 
 /* What the last upload planned: four-qubit blocks run fused, four-qubit
- * blocks run as primitive sweeps (the model's choice), five-qubit blocks, the
- * model's predicted device time of the four-qubit blocks' launches and its
- * prediction for the same blocks fused at the default width (microseconds; 0
- * without a model), and tiles[5] = fused four-qubit blocks per tile width
- * m = 4..8 (may be NULL). */
+ * blocks run as primitive sweeps (the model's choice), five-qubit blocks run
+ * fused (a five-qubit block the model splits counts as its four-qubit
+ * sub-blocks instead), the model's predicted device time of the blocks'
+ * launches and its prediction for the same blocks fused at the default width
+ * (microseconds; 0 without a model), and tiles[5] = fused four-qubit blocks
+ * per tile width m = 4..8 (may be NULL). */
 int dmx_plan_info(const dmx_sim* sim, uint32_t* fused_blocks, uint32_t* plain_blocks,
                   uint32_t* fused5_blocks, double* predicted_us, double* greedy_us,
                   uint32_t* tiles);

@@ -20,7 +20,7 @@ from __future__ import annotations
 
 import numpy as np
 
-F = 12
+F = 13
 H = 64
 P = H * F + H + H * H + H + H + 1
 

@@ -161,3 +161,55 @@ def test_learned_plan_judged_by_unfused(dm, fitted, n):
     assert abs(info["greedy_us"] / 1e3 / t_greedy - 1.0) < 0.35
     # the learned plan is not slower than greedy beyond timing noise
     assert t_learned <= 1.10 * t_greedy
+
+
+def test_five_qubit_launch_timing(dm):
+    """The five-qubit block sweep is timed like the other kinds, and bad
+    configurations are rejected before any launch."""
+    with dm.Sim(9) as s:
+        us = s.time_launch(dm.COST_FUSED5, 7, 0b11100110, 0, -1, 0, 5)
+        assert 0 < us < 1e4
+        us5 = s.time_launch(dm.COST_FUSED5, 5, 0x1F, 0, -1, 0, 5)
+        assert 0 < us5 < 1e4
+        for m, qmask in ((5, 0b11100110), (6, 0xF), (9, 0x1F)):
+            with pytest.raises(ValueError):
+                s.time_launch(dm.COST_FUSED5, m, qmask, 0, -1, 0, 1)
+
+
+@pytest.mark.parametrize("n", [8, 10])
+def test_learned_plan_with_five_qubit_blocks(dm, fitted, n):
+    """Blocks of up to five qubits: the learned plan, which may split a
+    five-qubit block into four-qubit ones, against the greedy plan and unfused
+    execution."""
+    import workloads as W
+    model = fitted[0]
+    table = W.random_mixed(n, 300, seed=200 + n)
+    print(f"cost: five-qubit learned plan vs greedy vs plain, n={n}", flush=True)
+
+    def make(method, mdl=None):
+        s = dm.Sim(n)
+        s.set_method(method)
+        s.set_fusion_max_k(5)
+        if mdl is not None:
+            s.set_cost_model(mdl)
+        s.append_table(table)
+        s.upload()
+        s.run()
+        return s
+
+    plain = make(dm.METHOD_PLAIN_LAUNCH)
+    greedy = make(dm.METHOD_FUSED)
+    learned = make(dm.METHOD_FUSED, model)
+    re0, im0 = plain.get_dm()
+    for s in (greedy, learned):
+        re, im = s.get_dm()
+        assert max(np.abs(re - re0).max(), np.abs(im - im0).max()) <= 1e-10
+    gi, info = greedy.plan_info(), learned.plan_info()
+    assert gi["fused5_blocks"] > 0
+    assert info["fused5_blocks"] <= gi["fused5_blocks"]
+    t_greedy = _time_runs(greedy)
+    t_learned = _time_runs(learned)
+    print(f"n={n}: greedy {t_greedy:.3f} ms plan {gi}; learned {t_learned:.3f} ms "
+          f"(predicted {info['predicted_us'] / 1e3:.3f}) plan {info}", flush=True)
+    assert abs(info["predicted_us"] / 1e3 / t_learned - 1.0) < 0.35
+    assert t_learned <= 1.10 * t_greedy
