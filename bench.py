@@ -33,7 +33,7 @@ HBM_PEAK_GBS = 8000.0
 # WRITE_SIZE runs of this workload scaled by a copy of known size
 # (profiles/k_fused4_n14_hbm_counters.md).
 TRAFFIC_RATIO = {"k_fused4": 8.591e9 / 8.590e9}
-METHODS = {"plain": 0, "plain_launch": 1, "fused": 2, "fused_learned": 2}
+METHODS = {"plain": 0, "plain_launch": 1, "fused": 2, "fused_learned": 2, "fused_learned5": 2}
 
 
 def log(msg):
@@ -162,13 +162,15 @@ def vqe_published(warmup=1, steps=5):
            "reference_ms": 249.3, "reference_hw": "1x NVIDIA V100 (README.md:47)"}
     import costdata
     model = costdata.default_model()
-    for mname in ("fused", "fused_learned", "plain"):
-        if mname == "fused_learned" and model is None:
+    for mname in ("fused", "fused_learned", "fused_learned5", "plain"):
+        if mname.startswith("fused_learned") and model is None:
             continue
         s = dmsim_amd.Sim(8)
         s.set_method(METHODS[mname])
-        if mname == "fused_learned":
+        if mname.startswith("fused_learned"):
             s.set_cost_model(model)
+        if mname == "fused_learned5":
+            s.set_fusion_max_k(5)
         s.append_qasm(src)
         s.upload()
         ms = []
@@ -204,7 +206,7 @@ def main():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--n", type=int, default=14)
-    ap.add_argument("--methods", default="fused,fused_learned,plain,plain_launch")
+    ap.add_argument("--methods", default="fused,fused_learned,fused_learned5,plain,plain_launch")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-published", action="store_true",
@@ -231,14 +233,16 @@ def main():
     for mname in args.methods.split(","):
         sim = dmsim_amd.Sim(n)
         sim.set_method(METHODS[mname])
-        if mname == "fused_learned":
+        if mname.startswith("fused_learned"):
             import costdata
             model = costdata.default_model()
             if model is None:
-                log("[bench] fused_learned: no shipped cost model, skipped")
+                log(f"[bench] {mname}: no shipped cost model, skipped")
                 sim.close()
                 continue
             sim.set_cost_model(model)
+        if mname == "fused_learned5":  # blocks of up to five qubits
+            sim.set_fusion_max_k(5)
         sim.append_table(table)
         sim.upload()
         if mname.startswith("fused"):
@@ -319,7 +323,8 @@ def main():
         line["published_comparison"] = vqe_published()
         pc = line["published_comparison"]
         log(f"[bench] vqe_uccsd_n8: fused {pc['fused_ms']:.2f} ms, learned "
-            f"{pc.get('fused_learned_ms', float('nan')):.2f} ms, plain {pc['plain_ms']:.2f} ms "
+            f"{pc.get('fused_learned_ms', float('nan')):.2f} ms, learned (5-qubit blocks) "
+            f"{pc.get('fused_learned5_ms', float('nan')):.2f} ms, plain {pc['plain_ms']:.2f} ms "
             f"(V100: 249.3 ms)")
     if not args.no_cpu_baseline:
         log("[bench] cpu baseline ...")

@@ -21,6 +21,7 @@
 #include <algorithm>
 #include <complex>
 #include <cstring>
+#include <map>
 
 #include "dmsim_amd.h"
 #include "engine.hpp"
@@ -182,24 +183,21 @@ void apply_embedded(std::vector<cd>& U, uint32_t qmask, const std::vector<uint32
     U.swap(out);
 }
 
-}  // namespace
+// A block under construction: qubit mask and member gates (circuit indices,
+// in application order).
+struct PB {
+    uint32_t mask;
+    std::vector<int> gates;
+};
+using Seq = std::vector<std::pair<int, int>>;  // (0, index into blocks) or (3, channel gate)
 
-int plan_blocks(const std::vector<Gate>& circuit, int n, int max_k, BlockPlan& out, bool& fusable)
+// Grouping 1, in program order: each gate joins the open block that must take
+// the fewest new qubits (the latest such block on ties) among the blocks at or
+// after the last one touching its qubits -- the blocks in between do not touch
+// them, so the gate commutes back to that point -- or opens a new block.
+void group_least_growth(const std::vector<Gate>& circuit, int kmax, std::vector<PB>& pbs, Seq& seq)
 {
-    out = BlockPlan{};
-    fusable = false;
-    if (n < 4) return DMX_OK;  // too few qubits for a 4-qubit block: plain sweeps
-    const int kmax = std::min(std::max(max_k, 1), std::min(5, n));
-    for (const Gate& g : circuit)
-        if (g.op == OP_SRN) return DMX_OK;  // not complex-linear: plain sweeps
-    fusable = true;
-    struct PB {
-        uint32_t mask;
-        std::vector<int> gates;
-    };
-    std::vector<PB> pbs;
-    std::vector<std::pair<int, int>> seq;  // (0, index into pbs) or (3, channel gate)
-    int frozen = 0;                        // blocks before a channel take no later gate
+    int frozen = 0;  // blocks before a channel take no later gate
     for (int gi = 0; gi < (int)circuit.size(); gi++) {
         const Gate& g = circuit[gi];
         if (g.op == OP_ID) continue;
@@ -216,7 +214,6 @@ int plan_blocks(const std::vector<Gate>& circuit, int n, int max_k, BlockPlan& o
                 jstar = b;
                 break;
             }
-        // least growth (new qubits the block must take), the latest block on ties
         int best = -1, best_g = 99;
         for (int b = std::max(jstar, frozen); b < (int)pbs.size(); b++) {
             const int have = __builtin_popcount(pbs[b].mask);
@@ -233,6 +230,187 @@ int plan_blocks(const std::vector<Gate>& circuit, int n, int max_k, BlockPlan& o
             seq.push_back({0, (int)pbs.size()});
             pbs.push_back({S, {gi}});
         }
+    }
+}
+
+// Grouping 2, over the dependency graph (gates on a common qubit keep their
+// order): blocks are built one at a time from the ready gates (those first in
+// line on every qubit they touch).  A block on qubit set Q absorbs, repeatedly,
+// every ready gate inside Q; Q starts as one ready gate's qubits and may take
+// a further ready gate's qubits while it stays within kmax.  Of all seeds and
+// growth paths (searched depth-first, memoised per Q) the block absorbing the
+// most gates is taken, the smaller Q on ties.
+class FrontierGrouper {
+public:
+    FrontierGrouper(const std::vector<Gate>& circuit, int n, int kmax)
+        : circuit_(circuit), n_(n), kmax_(kmax), lists_(n), head_(n, 0), mask_(circuit.size(), 0)
+    {
+    }
+
+    // Groups circuit indices seg (no channels, no identities) into pbs/seq.
+    void run(const std::vector<int>& seg, std::vector<PB>& pbs, Seq& seq)
+    {
+        for (auto& l : lists_) l.clear();
+        std::fill(head_.begin(), head_.end(), 0);
+        for (int gi : seg) {
+            const uint32_t m = gate_mask(circuit_[gi]);
+            mask_[gi] = m;
+            for (uint32_t b = m; b; b &= b - 1) lists_[__builtin_ctz(b)].push_back(gi);
+        }
+        size_t left = seg.size();
+        while (left > 0) {
+            memo_.clear();
+            explored_ = 0;
+            Grown best{0, -1};
+            for (int gi : ready(head_)) {
+                const Grown r = grow(mask_[gi]);
+                if (best.count < 0 || better(r, best)) best = r;
+            }
+            PB pb{0, {}};
+            std::vector<int> ptr;
+            closure(best.Q, pb.gates, ptr);
+            for (int gi : pb.gates) {
+                pb.mask |= mask_[gi];
+                for (uint32_t b = mask_[gi]; b; b &= b - 1) head_[__builtin_ctz(b)]++;
+            }
+            left -= pb.gates.size();
+            seq.push_back({0, (int)pbs.size()});
+            pbs.push_back(std::move(pb));
+        }
+    }
+
+private:
+    struct Grown {
+        uint32_t Q;
+        int count;  // gates the block on Q absorbs
+    };
+
+    static bool better(const Grown& a, const Grown& b)
+    {
+        if (a.count != b.count) return a.count > b.count;
+        return __builtin_popcount(a.Q) < __builtin_popcount(b.Q);
+    }
+
+    bool at_heads(int gi, const std::vector<int>& ptr) const
+    {
+        for (uint32_t b = mask_[gi]; b; b &= b - 1) {
+            const int q = __builtin_ctz(b);
+            if (ptr[q] >= (int)lists_[q].size() || lists_[q][ptr[q]] != gi) return false;
+        }
+        return true;
+    }
+
+    std::vector<int> ready(const std::vector<int>& ptr) const
+    {
+        std::vector<int> out;
+        for (int q = 0; q < n_; q++) {
+            if (ptr[q] >= (int)lists_[q].size()) continue;
+            const int gi = lists_[q][ptr[q]];
+            if (__builtin_ctz(mask_[gi]) == q && at_heads(gi, ptr)) out.push_back(gi);
+        }
+        return out;
+    }
+
+    // Gates absorbed from the current heads by a block on Q (in order), and
+    // the heads after them.
+    void closure(uint32_t Q, std::vector<int>& absorbed, std::vector<int>& ptr) const
+    {
+        ptr = head_;
+        for (bool changed = true; changed;) {
+            changed = false;
+            for (uint32_t b = Q; b; b &= b - 1) {
+                const int q = __builtin_ctz(b);
+                while (ptr[q] < (int)lists_[q].size()) {
+                    const int gi = lists_[q][ptr[q]];
+                    const uint32_t m = mask_[gi];
+                    if ((m & ~Q) || !at_heads(gi, ptr)) break;
+                    for (uint32_t c = m; c; c &= c - 1) ptr[__builtin_ctz(c)]++;
+                    absorbed.push_back(gi);
+                    changed = true;
+                }
+            }
+        }
+    }
+
+    Grown grow(uint32_t Q)
+    {
+        auto it = memo_.find(Q);
+        if (it != memo_.end()) return it->second;
+        std::vector<int> absorbed, ptr;
+        closure(Q, absorbed, ptr);
+        Grown best{Q, (int)absorbed.size()};
+        const int limit = std::max(kmax_, __builtin_popcount(Q));
+        // the search is bounded: past kMaxExplored growth states per block the
+        // best found so far is taken
+        if (__builtin_popcount(Q) < limit && ++explored_ <= kMaxExplored) {
+            for (int gi : ready(ptr)) {
+                const uint32_t Q2 = Q | mask_[gi];
+                if (Q2 == Q || __builtin_popcount(Q2) > limit) continue;
+                const Grown r = grow(Q2);
+                if (better(r, best)) best = r;
+            }
+        }
+        memo_[Q] = best;
+        return best;
+    }
+
+    static constexpr int kMaxExplored = 1024;
+    const std::vector<Gate>& circuit_;
+    const int n_, kmax_;
+    std::vector<std::vector<int>> lists_;  // per qubit: its gates in program order
+    std::vector<int> head_;                // per qubit: gates already grouped
+    std::vector<uint32_t> mask_;           // per circuit index: the gate's qubits
+    std::map<uint32_t, Grown> memo_;
+    int explored_ = 0;
+};
+
+void group_frontier(const std::vector<Gate>& circuit, int n, int kmax, std::vector<PB>& pbs,
+                    Seq& seq)
+{
+    FrontierGrouper fg(circuit, n, kmax);
+    std::vector<int> seg;
+    for (int gi = 0; gi < (int)circuit.size(); gi++) {
+        const Gate& g = circuit[gi];
+        if (g.op == OP_ID) continue;
+        if (g.op == OP_KRAUS) {  // a channel closes the segment: no gate crosses it
+            fg.run(seg, pbs, seq);
+            seg.clear();
+            seq.push_back({3, gi});
+            continue;
+        }
+        seg.push_back(gi);
+    }
+    fg.run(seg, pbs, seq);
+}
+
+// Sweeps a grouping costs: a five-qubit block counts as about 1.5 four-qubit
+// sweeps (its MFMA work is eight times a four-qubit block's per entry).
+double grouping_cost(const std::vector<PB>& pbs)
+{
+    double c = 0;
+    for (const PB& pb : pbs) c += __builtin_popcount(pb.mask) > 4 ? 1.5 : 1.0;
+    return c;
+}
+
+}  // namespace
+
+int plan_blocks(const std::vector<Gate>& circuit, int n, int max_k, BlockPlan& out, bool& fusable)
+{
+    out = BlockPlan{};
+    fusable = false;
+    if (n < 4) return DMX_OK;  // too few qubits for a 4-qubit block: plain sweeps
+    const int kmax = std::min(std::max(max_k, 1), std::min(5, n));
+    for (const Gate& g : circuit)
+        if (g.op == OP_SRN) return DMX_OK;  // not complex-linear: plain sweeps
+    fusable = true;
+    // both groupings; the one with fewer sweeps is kept (the first on ties)
+    std::vector<PB> pbs, pbs2;
+    Seq seq, seq2;
+    group_least_growth(circuit, kmax, pbs, seq);
+    group_frontier(circuit, n, kmax, pbs2, seq2);
+    if (grouping_cost(pbs2) < grouping_cost(pbs)) {
+        pbs.swap(pbs2);
+        seq.swap(seq2);
     }
     const uint32_t all = (1u << n) - 1;
     std::vector<std::pair<int, int>> where(pbs.size());  // pbs index -> order entry
@@ -494,6 +672,24 @@ int build_fused_plan(const std::vector<Gate>& circuit, int n, int max_k, CostMod
             plan->prims.insert(plan->prims.end(), plain[o.second].begin(), plain[o.second].end());
             plan->steps.push_back({ST_PLAIN, first, (int)plan->prims.size() - first});
         }
+    }
+    if (model && tile_bits <= 0 && !plan->blocks5.empty()) {
+        // Whole-plan check: five-qubit blocks pay off only when they replace
+        // enough four-qubit sweeps, which no single block's split can show.
+        FusedPlan* alt = nullptr;
+        PlanInfo ai;
+        rc = build_fused_plan(circuit, n, 4, model, 0, &alt, &ai);
+        if (rc) {
+            delete plan;
+            return rc;
+        }
+        if (alt && ai.predicted_us < pi.predicted_us) {
+            delete plan;
+            *out = alt;
+            if (info) *info = ai;
+            return DMX_OK;
+        }
+        delete alt;
     }
     *out = plan;
     if (info) *info = pi;

@@ -59,8 +59,10 @@ struct PlanInfo {
 // Plan of a circuit for the fused executor.  With a cost model, each block is
 // priced as a fused launch at every tile width and as its primitive sweeps
 // (a five-qubit block: as its member gates re-planned into four-qubit
-// blocks), and the cheapest is taken; without one, every block is fused at
-// the default tile width (the greedy plan).
+// blocks), and the cheapest is taken; a plan with five-qubit blocks is kept
+// only if predicted faster than the four-qubit plan of the same circuit.
+// Without a model, every block is fused at the default tile width (the
+// greedy plan).
 // tile_bits > 0 fixes every fused block's tile width where that width is
 // admissible for the block (tile_bits_ok), and takes precedence over the model.
 int build_fused_plan(const std::vector<Gate>& circuit, int n, int max_k, CostModel* model,
