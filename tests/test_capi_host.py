@@ -114,3 +114,26 @@ def test_cpp_faces_build(tmp_path, name):
     interface (namespace DmSim, DmSimBackend) compile and link against the
     header-only faces of the C ABI."""
     assert os.path.exists(build_cpp_facade_example(tmp_path, name))
+
+
+def test_planner_grouping_is_valid(tmp_path):
+    """The fused planner's grouping (dmx::plan_blocks, an exported C++ symbol of
+    the library) on 108 seeded random circuits of 4..12 qubits with one- to
+    five-qubit gates, identities and channels, blocks of at most four and five
+    qubits: every gate in exactly one block, per-qubit program order kept, no
+    gate across a channel, members within their block (tests/cpp/plan_groups.cpp).
+    The internal headers include HIP's, so hipcc compiles it (host code only)."""
+    import shutil
+    import subprocess
+    hipcc = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
+    if not os.path.exists(hipcc):
+        pytest.skip("hipcc not found")
+    exe = os.path.join(str(tmp_path), "plan_groups")
+    pkg = os.path.join(ROOT, "dm-sim_amd")
+    cmd = [hipcc, "-std=c++17", "-O2", "-I", os.path.join(pkg, "csrc"), "-I",
+           os.path.join(ROOT, "include"), os.path.join(ROOT, "tests", "cpp", "plan_groups.cpp"),
+           "-L", pkg, "-ldmsim_amd", f"-Wl,-rpath,{pkg}", "-o", exe]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    r = subprocess.run([exe], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0 and r.stdout.strip() == "108 cases OK", r.stdout
