@@ -237,13 +237,16 @@ void group_least_growth(const std::vector<Gate>& circuit, int kmax, std::vector<
 // order): blocks are built one at a time from the ready gates (those first in
 // line on every qubit they touch).  A block on qubit set Q absorbs, repeatedly,
 // every ready gate inside Q; Q starts as one ready gate's qubits and may take
-// a further ready gate's qubits while it stays within kmax.  Of all seeds and
-// growth paths (searched depth-first, memoised per Q) the block absorbing the
-// most gates is taken, the smaller Q on ties.
+// a further ready gate's qubits while it stays within kmax.  Every Q reached
+// this way (depth-first, memoised, at most kMaxExplored per search) is a
+// candidate, scored by the gates it absorbs: 100 per multi-qubit gate, 1 per
+// one-qubit gate (those join almost any block later).  The kLook best
+// candidates are compared by their score plus the best score of the block
+// that would follow them; the smaller Q wins ties.
 class FrontierGrouper {
 public:
     FrontierGrouper(const std::vector<Gate>& circuit, int n, int kmax)
-        : circuit_(circuit), n_(n), kmax_(kmax), lists_(n), head_(n, 0), mask_(circuit.size(), 0)
+        : circuit_(circuit), n_(n), kmax_(kmax), lists_(n), mask_(circuit.size(), 0)
     {
     }
 
@@ -251,28 +254,45 @@ public:
     void run(const std::vector<int>& seg, std::vector<PB>& pbs, Seq& seq)
     {
         for (auto& l : lists_) l.clear();
-        std::fill(head_.begin(), head_.end(), 0);
         for (int gi : seg) {
             const uint32_t m = gate_mask(circuit_[gi]);
             mask_[gi] = m;
             for (uint32_t b = m; b; b &= b - 1) lists_[__builtin_ctz(b)].push_back(gi);
         }
+        // lookahead costs about kLook + 1 searches per block: skipped for very
+        // long segments to bound planning time
+        const int look = seg.size() <= kLookMaxGates ? kLook : 0;
+        std::vector<int> head(n_, 0);
         size_t left = seg.size();
         while (left > 0) {
-            memo_.clear();
-            explored_ = 0;
-            Grown best{0, -1};
-            for (int gi : ready(head_)) {
-                const Grown r = grow(mask_[gi]);
-                if (best.count < 0 || better(r, best)) best = r;
+            std::vector<Cand> c1;
+            search(head, c1);
+            std::sort(c1.begin(), c1.end(), ranked);
+            const Cand* best = &c1[0];
+            if (look > 0 && c1[0].count < (int)left) {
+                long best_total = -1;
+                for (int i = 0; i < (int)c1.size() && i < look; i++) {
+                    long total = c1[i].score;
+                    if (c1[i].count == (int)left) {
+                        total += 1L << 40;  // finishes the segment
+                    } else {
+                        std::vector<Cand> c2;
+                        search(c1[i].ptr, c2);
+                        long s2 = 0;
+                        for (const Cand& c : c2) s2 = std::max(s2, c.score);
+                        total += s2;
+                    }
+                    if (total > best_total) {
+                        best_total = total;
+                        best = &c1[i];
+                    }
+                }
             }
             PB pb{0, {}};
             std::vector<int> ptr;
-            closure(best.Q, pb.gates, ptr);
-            for (int gi : pb.gates) {
-                pb.mask |= mask_[gi];
-                for (uint32_t b = mask_[gi]; b; b &= b - 1) head_[__builtin_ctz(b)]++;
-            }
+            closure(best->Q, head, pb.gates, ptr);
+            for (int gi : pb.gates) pb.mask |= mask_[gi];
+            head.swap(ptr);
             left -= pb.gates.size();
             seq.push_back({0, (int)pbs.size()});
             pbs.push_back(std::move(pb));
@@ -280,15 +300,19 @@ public:
     }
 
 private:
-    struct Grown {
+    struct Cand {
         uint32_t Q;
-        int count;  // gates the block on Q absorbs
+        long score;             // 100 per multi-qubit gate absorbed, 1 per one-qubit gate
+        int count;              // gates absorbed
+        std::vector<int> ptr;   // per-qubit heads after them
     };
 
-    static bool better(const Grown& a, const Grown& b)
+    static bool ranked(const Cand& a, const Cand& b)
     {
-        if (a.count != b.count) return a.count > b.count;
-        return __builtin_popcount(a.Q) < __builtin_popcount(b.Q);
+        if (a.score != b.score) return a.score > b.score;
+        const int wa = __builtin_popcount(a.Q), wb = __builtin_popcount(b.Q);
+        if (wa != wb) return wa < wb;
+        return a.Q < b.Q;
     }
 
     bool at_heads(int gi, const std::vector<int>& ptr) const
@@ -311,11 +335,12 @@ private:
         return out;
     }
 
-    // Gates absorbed from the current heads by a block on Q (in order), and
-    // the heads after them.
-    void closure(uint32_t Q, std::vector<int>& absorbed, std::vector<int>& ptr) const
+    // Gates a block on Q absorbs from heads `from` (in order), and the heads
+    // after them.
+    void closure(uint32_t Q, const std::vector<int>& from, std::vector<int>& absorbed,
+                 std::vector<int>& ptr) const
     {
-        ptr = head_;
+        ptr = from;
         for (bool changed = true; changed;) {
             changed = false;
             for (uint32_t b = Q; b; b &= b - 1) {
@@ -332,36 +357,43 @@ private:
         }
     }
 
-    Grown grow(uint32_t Q)
+    // Every candidate block reachable from heads `from`.
+    void search(const std::vector<int>& from, std::vector<Cand>& out) const
     {
-        auto it = memo_.find(Q);
-        if (it != memo_.end()) return it->second;
-        std::vector<int> absorbed, ptr;
-        closure(Q, absorbed, ptr);
-        Grown best{Q, (int)absorbed.size()};
-        const int limit = std::max(kmax_, __builtin_popcount(Q));
-        // the search is bounded: past kMaxExplored growth states per block the
-        // best found so far is taken
-        if (__builtin_popcount(Q) < limit && ++explored_ <= kMaxExplored) {
-            for (int gi : ready(ptr)) {
-                const uint32_t Q2 = Q | mask_[gi];
-                if (Q2 == Q || __builtin_popcount(Q2) > limit) continue;
-                const Grown r = grow(Q2);
-                if (better(r, best)) best = r;
+        std::map<uint32_t, int> seen;  // Q -> index into out
+        std::vector<uint32_t> stack;
+        for (int gi : ready(from)) stack.push_back(mask_[gi]);
+        std::reverse(stack.begin(), stack.end());
+        while (!stack.empty() && (int)out.size() < kMaxExplored) {
+            const uint32_t Q = stack.back();
+            stack.pop_back();
+            if (seen.count(Q)) continue;
+            Cand c{Q, 0, 0, {}};
+            std::vector<int> absorbed;
+            closure(Q, from, absorbed, c.ptr);
+            for (int gi : absorbed) c.score += __builtin_popcount(mask_[gi]) > 1 ? 100 : 1;
+            c.count = (int)absorbed.size();
+            seen[Q] = (int)out.size();
+            const int limit = std::max(kmax_, __builtin_popcount(Q));
+            if (__builtin_popcount(Q) < limit) {
+                std::vector<int> r = ready(c.ptr);
+                for (auto it = r.rbegin(); it != r.rend(); ++it) {
+                    const uint32_t Q2 = Q | mask_[*it];
+                    if (Q2 != Q && __builtin_popcount(Q2) <= limit && !seen.count(Q2))
+                        stack.push_back(Q2);
+                }
             }
+            out.push_back(std::move(c));
         }
-        memo_[Q] = best;
-        return best;
     }
 
     static constexpr int kMaxExplored = 1024;
+    static constexpr int kLook = 4;
+    static constexpr size_t kLookMaxGates = 1u << 18;
     const std::vector<Gate>& circuit_;
     const int n_, kmax_;
     std::vector<std::vector<int>> lists_;  // per qubit: its gates in program order
-    std::vector<int> head_;                // per qubit: gates already grouped
     std::vector<uint32_t> mask_;           // per circuit index: the gate's qubits
-    std::map<uint32_t, Grown> memo_;
-    int explored_ = 0;
 };
 
 void group_frontier(const std::vector<Gate>& circuit, int n, int kmax, std::vector<PB>& pbs,
