@@ -22,6 +22,8 @@
 #include <complex>
 #include <cstring>
 #include <map>
+#include <queue>
+#include <tuple>
 
 #include "dmsim_amd.h"
 #include "engine.hpp"
@@ -233,20 +235,62 @@ void group_least_growth(const std::vector<Gate>& circuit, int kmax, std::vector<
     }
 }
 
-// Grouping 2, over the dependency graph (gates on a common qubit keep their
-// order): blocks are built one at a time from the ready gates (those first in
-// line on every qubit they touch).  A block on qubit set Q absorbs, repeatedly,
-// every ready gate inside Q; Q starts as one ready gate's qubits and may take
-// a further ready gate's qubits while it stays within kmax.  Every Q reached
-// this way (depth-first, memoised, at most kMaxExplored per search) is a
-// candidate, scored by the gates it absorbs: 100 per multi-qubit gate, 1 per
-// one-qubit gate (those join almost any block later).  The kLook best
-// candidates are compared by their score plus the best score of the block
-// that would follow them; the smaller Q wins ties.
+// Per-qubit commutation classes of a gate, from its own matrix: bit q of zm
+// (xm) is set when the gate is block-diagonal in the Z (X) basis of qubit q,
+// i.e. of the form sum_k P_k(q) (x) G_k with P_k the projectors of that basis.
+// Two gates whose every shared qubit has a common class commute: on each
+// shared qubit both are sums over the same projectors, and what remains acts
+// on disjoint qubits.
+void commute_classes(const Gate& g, uint32_t& zm, uint32_t& xm)
+{
+    zm = xm = 0;
+    int k = 0;
+    std::vector<uint32_t> qs;
+    std::vector<double> re, im;
+    gate_matrix(g, k, qs, re, im);
+    const int D = 1 << k;
+    constexpr double kTol = 1e-12;
+    auto at = [&](int r, int c) { return cd(re[(size_t)r * D + c], im[(size_t)r * D + c]); };
+    for (int j = 0; j < k; j++) {
+        const int b = 1 << (k - 1 - j);  // matrix index bit of operand j
+        bool z = true, x = true;
+        for (int r = 0; r < D && (z || x); r++)
+            for (int c = 0; c < D && (z || x); c++) {
+                if (((r ^ c) & b) == 0) continue;
+                if (std::abs(at(r, c)) > kTol) z = false;
+                // (H_b M H_b)[r][c] = 1/2 sum_{a,d} (-1)^(r_b a + d c_b) M[r_a][c_d]
+                const int r0 = r & ~b, c0 = c & ~b;
+                const double sr = (r & b) ? -1.0 : 1.0, sc = (c & b) ? -1.0 : 1.0;
+                const cd h = 0.5 * (at(r0, c0) + sc * at(r0, c0 | b) + sr * at(r0 | b, c0) +
+                                    sr * sc * at(r0 | b, c0 | b));
+                if (std::abs(h) > kTol) x = false;
+            }
+        if (z) zm |= 1u << qs[j];
+        if (x) xm |= 1u << qs[j];
+    }
+}
+
+// Grouping 2, over the gates' dependency order: a gate depends on every
+// earlier gate that shares a qubit and does not commute with it (classes
+// above).  Blocks are built one at a time from the ready gates (all their
+// dependencies grouped).  A block on qubit set Q absorbs, repeatedly, every
+// ready gate inside Q; Q starts as one ready gate's qubits and may take a
+// further ready gate's qubits while it stays within kmax.  The absorption
+// order is an order of the circuit's gates that differs from program order
+// only by swaps of commuting gates, so the blocks' product is the circuit's.
+// Every Q reached this way (depth-first, memoised, at most kMaxExplored per
+// search) is a candidate, scored by the gates it absorbs: 100 per multi-qubit
+// gate, 1 per one-qubit gate (those join almost any block later).  The kLook
+// best candidates are compared by their score plus the best score of the
+// block that would follow them; the smaller Q wins ties.  A gate is looked
+// for at most kWindow places past the first ungrouped gate of each qubit.
 class FrontierGrouper {
 public:
     FrontierGrouper(const std::vector<Gate>& circuit, int n, int kmax)
-        : circuit_(circuit), n_(n), kmax_(kmax), lists_(n), mask_(circuit.size(), 0)
+        : circuit_(circuit), n_(n), kmax_(kmax), lists_(n), head_(n, 0),
+          mask_(circuit.size(), 0), zm_(circuit.size(), 0), xm_(circuit.size(), 0),
+          pos0_(circuit.size(), 0), done_(circuit.size(), 0), mark_(circuit.size(), 0),
+          listed_(circuit.size(), 0), base_(n), work_(n)
     {
     }
 
@@ -254,30 +298,39 @@ public:
     void run(const std::vector<int>& seg, std::vector<PB>& pbs, Seq& seq)
     {
         for (auto& l : lists_) l.clear();
+        std::fill(head_.begin(), head_.end(), 0);
+        pos_.clear();
         for (int gi : seg) {
             const uint32_t m = gate_mask(circuit_[gi]);
             mask_[gi] = m;
-            for (uint32_t b = m; b; b &= b - 1) lists_[__builtin_ctz(b)].push_back(gi);
+            classes_of(circuit_[gi], zm_[gi], xm_[gi]);
+            pos0_[gi] = (int)pos_.size();
+            for (uint32_t b = m; b; b &= b - 1) {
+                const int q = __builtin_ctz(b);
+                pos_.push_back((int)lists_[q].size());
+                lists_[q].push_back(gi);
+            }
         }
         // lookahead costs about kLook + 1 searches per block: skipped for very
         // long segments to bound planning time
-        const int look = seg.size() <= kLookMaxGates ? kLook : 0;
-        std::vector<int> head(n_, 0);
+        const bool small = seg.size() <= kLookMaxGates;
+        const int look = small ? kLook : 0;
+        const int cap = small ? kMaxExplored : kMaxExploredLong;
         size_t left = seg.size();
         while (left > 0) {
             std::vector<Cand> c1;
-            search(head, c1);
+            search({}, cap, c1);
             std::sort(c1.begin(), c1.end(), ranked);
             const Cand* best = &c1[0];
-            if (look > 0 && c1[0].count < (int)left) {
+            if (look > 0 && c1[0].taken.size() < left) {
                 long best_total = -1;
                 for (int i = 0; i < (int)c1.size() && i < look; i++) {
                     long total = c1[i].score;
-                    if (c1[i].count == (int)left) {
+                    if (c1[i].taken.size() == left) {
                         total += 1L << 40;  // finishes the segment
                     } else {
                         std::vector<Cand> c2;
-                        search(c1[i].ptr, c2);
+                        search(c1[i].taken, kMaxExplored2, c2);
                         long s2 = 0;
                         for (const Cand& c : c2) s2 = std::max(s2, c.score);
                         total += s2;
@@ -288,11 +341,15 @@ public:
                     }
                 }
             }
-            PB pb{0, {}};
-            std::vector<int> ptr;
-            closure(best->Q, head, pb.gates, ptr);
-            for (int gi : pb.gates) pb.mask |= mask_[gi];
-            head.swap(ptr);
+            PB pb{0, best->taken};
+            for (int gi : pb.gates) {
+                pb.mask |= mask_[gi];
+                done_[gi] = 1;
+            }
+            for (uint32_t b = pb.mask; b; b &= b - 1) {
+                const int q = __builtin_ctz(b);
+                while (head_[q] < (int)lists_[q].size() && done_[lists_[q][head_[q]]]) head_[q]++;
+            }
             left -= pb.gates.size();
             seq.push_back({0, (int)pbs.size()});
             pbs.push_back(std::move(pb));
@@ -302,10 +359,13 @@ public:
 private:
     struct Cand {
         uint32_t Q;
-        long score;             // 100 per multi-qubit gate absorbed, 1 per one-qubit gate
-        int count;              // gates absorbed
-        std::vector<int> ptr;   // per-qubit heads after them
+        long score;               // 100 per multi-qubit gate absorbed, 1 per one-qubit gate
+        std::vector<int> taken;   // the search's starting set, then the gates absorbed
     };
+
+    // Flags of the ungrouped gates before a place in a qubit's list: some
+    // gate, one not Z-diagonal there, one not X-diagonal, one neither.
+    enum : uint8_t { F_ANY = 1, F_NZ = 2, F_NX = 4, F_GEN = 8 };
 
     static bool ranked(const Cand& a, const Cand& b)
     {
@@ -315,85 +375,206 @@ private:
         return a.Q < b.Q;
     }
 
-    bool at_heads(int gi, const std::vector<int>& ptr) const
+    void classes_of(const Gate& g, uint32_t& zm, uint32_t& xm)
     {
-        for (uint32_t b = mask_[gi]; b; b &= b - 1) {
+        if (g.op == OP_MAT) {
+            commute_classes(g, zm, xm);
+            return;
+        }
+        const auto key = std::make_tuple((int)g.op, g.theta, g.phi, g.lambda);
+        auto it = class_cache_.find(key);
+        if (it == class_cache_.end()) {
+            // classes with operand j on qubit j, then kept per operand position
+            Gate p = g;
+            const int k = op_arity(g.op);
+            for (int j = 0; j < k; j++) p.qb[j] = (uint32_t)j;
+            uint32_t z = 0, x = 0;
+            commute_classes(p, z, x);
+            it = class_cache_.emplace(key, std::make_pair(z, x)).first;
+        }
+        zm = xm = 0;
+        const int k = op_arity(g.op);
+        for (int j = 0; j < k; j++) {
+            if ((it->second.first >> j) & 1u) zm |= 1u << g.qb[j];
+            if ((it->second.second >> j) & 1u) xm |= 1u << g.qb[j];
+        }
+    }
+
+    bool taken(int gi) const { return done_[gi] || mark_[gi] == stamp_; }
+
+    long weight(int gi) const { return __builtin_popcount(mask_[gi]) > 1 ? 100 : 1; }
+
+    int window_end(int q) const
+    {
+        return std::min((int)lists_[q].size(), head_[q] + kWindow + 1);
+    }
+
+    // f[i] = flags of the ungrouped gates before window place i of qubit q.
+    void prefix(int q, std::vector<uint8_t>& f) const
+    {
+        const int end = window_end(q);
+        f.assign(end - head_[q] + 1, 0);
+        uint8_t acc = 0;
+        for (int i = head_[q]; i < end; i++) {
+            f[i - head_[q]] = acc;
+            const int h = lists_[q][i];
+            if (!taken(h)) {
+                const bool z = (zm_[h] >> q) & 1u, x = (xm_[h] >> q) & 1u;
+                acc |= F_ANY | (z ? 0 : F_NZ) | (x ? 0 : F_NX) | (z || x ? 0 : F_GEN);
+            }
+        }
+        f[end - head_[q]] = acc;
+    }
+
+    // No ungrouped gate before gi on any of its qubits fails to commute with
+    // it (flags from work_).
+    bool ready(int gi) const
+    {
+        int j = 0;
+        for (uint32_t b = mask_[gi]; b; b &= b - 1, j++) {
             const int q = __builtin_ctz(b);
-            if (ptr[q] >= (int)lists_[q].size() || lists_[q][ptr[q]] != gi) return false;
+            const int p = pos_[pos0_[gi] + j] - head_[q];
+            if (p > kWindow) return false;
+            const uint8_t acc = work_[q][p];
+            const bool z = (zm_[gi] >> q) & 1u, x = (xm_[gi] >> q) & 1u;
+            const uint8_t stop = z && x ? F_GEN : z ? F_NZ : x ? F_NX : F_ANY;
+            if (acc & stop) return false;
         }
         return true;
     }
 
-    std::vector<int> ready(const std::vector<int>& ptr) const
+    bool general_on(int h, int q) const { return (((zm_[h] | xm_[h]) >> q) & 1u) == 0; }
+
+    // Ready, untaken gates with a qubit in `qs`, appended once each (listed_
+    // stamps).
+    void ready_on(uint32_t qs, std::vector<int>& out)
     {
-        std::vector<int> out;
-        for (int q = 0; q < n_; q++) {
-            if (ptr[q] >= (int)lists_[q].size()) continue;
-            const int gi = lists_[q][ptr[q]];
-            if (__builtin_ctz(mask_[gi]) == q && at_heads(gi, ptr)) out.push_back(gi);
+        for (uint32_t b = qs; b; b &= b - 1) {
+            const int q = __builtin_ctz(b);
+            const int end = window_end(q);
+            for (int i = head_[q]; i < end; i++) {
+                const int h = lists_[q][i];
+                if (taken(h)) continue;
+                if (listed_[h] != stamp_ && ready(h)) {
+                    listed_[h] = stamp_;
+                    out.push_back(h);
+                }
+                if (general_on(h, q)) break;  // nothing later on q passes it
+            }
         }
-        return out;
     }
 
-    // Gates a block on Q absorbs from heads `from` (in order), and the heads
-    // after them.
-    void closure(uint32_t Q, const std::vector<int>& from, std::vector<int>& absorbed,
-                 std::vector<int>& ptr) const
+    // Absorbs, under the current marks, every ready gate inside Q; appends
+    // them to `absorbed` in absorption order and marks them.
+    void closure(uint32_t Q, std::vector<int>& absorbed)
     {
-        ptr = from;
         for (bool changed = true; changed;) {
             changed = false;
+            for (uint32_t b = Q; b; b &= b - 1) prefix(__builtin_ctz(b), work_[__builtin_ctz(b)]);
             for (uint32_t b = Q; b; b &= b - 1) {
                 const int q = __builtin_ctz(b);
-                while (ptr[q] < (int)lists_[q].size()) {
-                    const int gi = lists_[q][ptr[q]];
-                    const uint32_t m = mask_[gi];
-                    if ((m & ~Q) || !at_heads(gi, ptr)) break;
-                    for (uint32_t c = m; c; c &= c - 1) ptr[__builtin_ctz(c)]++;
-                    absorbed.push_back(gi);
-                    changed = true;
+                const int end = window_end(q);
+                for (int i = head_[q]; i < end; i++) {
+                    const int h = lists_[q][i];
+                    if (taken(h)) continue;
+                    if ((mask_[h] & ~Q) == 0 && ready(h)) {
+                        mark_[h] = stamp_;
+                        absorbed.push_back(h);
+                        changed = true;
+                    } else if (general_on(h, q)) {
+                        break;
+                    }
                 }
             }
         }
     }
 
-    // Every candidate block reachable from heads `from`.
-    void search(const std::vector<int>& from, std::vector<Cand>& out) const
+    // Every candidate block reachable once the gates `from` are grouped.
+    // Children Q | (a ready gate's qubits) continue from their parent's
+    // absorbed gates (absorption only grows with Q).
+    void search(const std::vector<int>& from, int cap, std::vector<Cand>& out)
     {
-        std::map<uint32_t, int> seen;  // Q -> index into out
-        std::vector<uint32_t> stack;
-        for (int gi : ready(from)) stack.push_back(mask_[gi]);
-        std::reverse(stack.begin(), stack.end());
-        while (!stack.empty() && (int)out.size() < kMaxExplored) {
-            const uint32_t Q = stack.back();
-            stack.pop_back();
+        const uint32_t all = n_ >= 32 ? ~0u : (1u << n_) - 1;
+        ++stamp_;
+        for (int gi : from) mark_[gi] = stamp_;
+        for (int q = 0; q < n_; q++) prefix(q, base_[q]);
+        work_ = base_;
+        std::vector<int> base_ready;
+        ready_on(all, base_ready);
+        std::map<uint32_t, int> seen;
+        // best first: (priority = parent's score + the joining gate's weight, Q, parent)
+        struct Node {
+            long prio;
+            uint32_t Q;
+            int parent;
+        };
+        auto worse = [](const Node& a, const Node& b) {
+            if (a.prio != b.prio) return a.prio < b.prio;
+            const int wa = __builtin_popcount(a.Q), wb = __builtin_popcount(b.Q);
+            if (wa != wb) return wa > wb;
+            return a.Q > b.Q;
+        };
+        std::priority_queue<Node, std::vector<Node>, decltype(worse)> open(worse);
+        for (int gi : base_ready) open.push({weight(gi), mask_[gi], -1});
+        uint32_t touched = 0;  // qubits whose work_ flags differ from base_
+        while (!open.empty() && (int)out.size() < cap) {
+            const Node nd = open.top();
+            open.pop();
+            const uint32_t Q = nd.Q;
+            const int parent = nd.parent;
             if (seen.count(Q)) continue;
-            Cand c{Q, 0, 0, {}};
-            std::vector<int> absorbed;
-            closure(Q, from, absorbed, c.ptr);
-            for (int gi : absorbed) c.score += __builtin_popcount(mask_[gi]) > 1 ? 100 : 1;
-            c.count = (int)absorbed.size();
+            Cand c{Q, parent < 0 ? 0 : out[parent].score,
+                   parent < 0 ? from : out[parent].taken};
+            ++stamp_;
+            for (int gi : c.taken) mark_[gi] = stamp_;
+            for (uint32_t b = touched; b; b &= b - 1) work_[__builtin_ctz(b)] = base_[__builtin_ctz(b)];
+            touched = Q;
+            const size_t before = c.taken.size();
+            closure(Q, c.taken);
+            for (size_t i = before; i < c.taken.size(); i++) c.score += weight(c.taken[i]);
             seen[Q] = (int)out.size();
             const int limit = std::max(kmax_, __builtin_popcount(Q));
             if (__builtin_popcount(Q) < limit) {
-                std::vector<int> r = ready(c.ptr);
-                for (auto it = r.rbegin(); it != r.rend(); ++it) {
-                    const uint32_t Q2 = Q | mask_[*it];
+                // ready gates now: those of the search's start off Q (their
+                // qubits saw no change), and a rescan of Q's qubits
+                for (uint32_t b = Q; b; b &= b - 1) prefix(__builtin_ctz(b), work_[__builtin_ctz(b)]);
+                std::vector<int> r;
+                for (int gi : base_ready)
+                    if (!taken(gi) && (mask_[gi] & Q) == 0) {
+                        listed_[gi] = stamp_;
+                        r.push_back(gi);
+                    }
+                ready_on(Q, r);
+                for (int gi : r) {
+                    const uint32_t Q2 = Q | mask_[gi];
                     if (Q2 != Q && __builtin_popcount(Q2) <= limit && !seen.count(Q2))
-                        stack.push_back(Q2);
+                        open.push({c.score + weight(gi), Q2, (int)out.size()});
                 }
             }
             out.push_back(std::move(c));
         }
     }
 
-    static constexpr int kMaxExplored = 1024;
+    static constexpr int kMaxExplored = 128;
+    static constexpr int kMaxExplored2 = 32;  // lookahead searches
     static constexpr int kLook = 4;
-    static constexpr size_t kLookMaxGates = 1u << 18;
+    static constexpr int kWindow = 64;
+    static constexpr int kMaxExploredLong = 32;  // segments above kLookMaxGates
+    static constexpr size_t kLookMaxGates = 1u << 16;
     const std::vector<Gate>& circuit_;
     const int n_, kmax_;
     std::vector<std::vector<int>> lists_;  // per qubit: its gates in program order
-    std::vector<uint32_t> mask_;           // per circuit index: the gate's qubits
+    std::vector<int> head_;                // per qubit: first gate not yet grouped
+    std::vector<uint32_t> mask_, zm_, xm_; // per circuit index: qubits, commutation classes
+    std::vector<int> pos0_, pos_;          // gate's place in each of its qubits' lists
+    std::vector<uint8_t> done_;            // grouped into a committed block
+    std::vector<uint64_t> mark_;           // == stamp_: taken in the current evaluation
+    std::vector<uint64_t> listed_;         // == stamp_: already in a ready list
+    uint64_t stamp_ = 0;
+    std::vector<std::vector<uint8_t>> base_, work_;  // per qubit window flags (prefix)
+    // classes per operand position (bit j = operand j) of the gates seen, by
+    // op and angles (matrix gates are not cached)
+    std::map<std::tuple<int, double, double, double>, std::pair<uint32_t, uint32_t>> class_cache_;
 };
 
 void group_frontier(const std::vector<Gate>& circuit, int n, int kmax, std::vector<PB>& pbs,
@@ -708,6 +889,8 @@ int build_fused_plan(const std::vector<Gate>& circuit, int n, int max_k, CostMod
     if (model && tile_bits <= 0 && !plan->blocks5.empty()) {
         // Whole-plan check: five-qubit blocks pay off only when they replace
         // enough four-qubit sweeps, which no single block's split can show.
+        // The requested width stays unless the narrower plan is predicted
+        // faster by the margin.
         FusedPlan* alt = nullptr;
         PlanInfo ai;
         rc = build_fused_plan(circuit, n, 4, model, 0, &alt, &ai);
@@ -715,7 +898,7 @@ int build_fused_plan(const std::vector<Gate>& circuit, int n, int max_k, CostMod
             delete plan;
             return rc;
         }
-        if (alt && ai.predicted_us < pi.predicted_us) {
+        if (alt && ai.predicted_us < kMargin * pi.predicted_us) {
             delete plan;
             *out = alt;
             if (info) *info = ai;

@@ -1,10 +1,13 @@
 // Validity of the fused planner's grouping (dmx::plan_blocks) on seeded random
-// circuits, checked on the host: every gate lands in exactly one block, the
-// gates on each qubit keep their program order in the blocks' execution
-// order, no gate crosses a channel, and every block holds its members'
-// qubits within its width.  Prints "<cases> cases OK", or the first violation
-// and exits nonzero.
+// circuits, checked on the host: every gate lands in exactly one block, every
+// block holds its members' qubits within its width, no gate crosses a
+// channel, and the blocks' execution order (which may swap commuting gates)
+// applies the same operator as program order -- both orders are applied to a
+// random state with the gates' own matrices (dmx::gate_matrix) and compared.
+// Prints "<cases> cases OK", or the first violation and exits nonzero.
 #include <algorithm>
+#include <cmath>
+#include <complex>
 #include <cstdio>
 #include <cstdlib>
 #include <random>
@@ -14,6 +17,7 @@
 #include "lower.hpp"
 
 using dmx::Gate;
+using cd = std::complex<double>;
 
 static int fail(const char* what, int n, int k, int seed)
 {
@@ -21,9 +25,45 @@ static int fail(const char* what, int n, int k, int seed)
     return 1;
 }
 
+// Random d x d unitary (Gram-Schmidt on Gaussian columns), or a random
+// diagonal of phases.
+static void random_matrix(std::mt19937& rng, int d, bool diagonal, std::vector<double>& re,
+                          std::vector<double>& im)
+{
+    std::normal_distribution<double> g(0.0, 1.0);
+    std::vector<cd> M((size_t)d * d, 0.0);
+    if (diagonal) {
+        for (int i = 0; i < d; i++) M[(size_t)i * d + i] = std::polar(1.0, g(rng));
+    } else {
+        for (int c = 0; c < d; c++) {
+            std::vector<cd> v(d);
+            for (int r = 0; r < d; r++) v[r] = cd(g(rng), g(rng));
+            for (int p = 0; p < c; p++) {
+                cd dot = 0.0;
+                for (int r = 0; r < d; r++) dot += std::conj(M[(size_t)r * d + p]) * v[r];
+                for (int r = 0; r < d; r++) v[r] -= dot * M[(size_t)r * d + p];
+            }
+            double nrm = 0;
+            for (int r = 0; r < d; r++) nrm += std::norm(v[r]);
+            nrm = std::sqrt(nrm);
+            for (int r = 0; r < d; r++) M[(size_t)r * d + c] = v[r] / nrm;
+        }
+    }
+    re.resize(M.size());
+    im.resize(M.size());
+    for (size_t i = 0; i < M.size(); i++) re[i] = M[i].real(), im[i] = M[i].imag();
+}
+
 static std::vector<Gate> random_circuit(int n, int count, int seed, bool channels)
 {
+    static const int kOps[] = {dmx::OP_U3, dmx::OP_RZ, dmx::OP_U1, dmx::OP_RX, dmx::OP_X,
+                               dmx::OP_H, dmx::OP_T, dmx::OP_CX, dmx::OP_CX, dmx::OP_CX,
+                               dmx::OP_CZ, dmx::OP_CU1, dmx::OP_RZZ, dmx::OP_RXX, dmx::OP_CRX,
+                               dmx::OP_CCX, dmx::OP_C3X, dmx::OP_CSWAP, dmx::OP_ID,
+                               dmx::OP_MAT, dmx::OP_MAT};
+    constexpr int kNumOps = sizeof(kOps) / sizeof(kOps[0]);
     std::mt19937 rng((uint32_t)seed);
+    std::uniform_real_distribution<double> ang(-3.2, 3.2);
     std::vector<Gate> c;
     auto pick = [&](int k, uint32_t* qb) {
         std::vector<int> q(n);
@@ -33,32 +73,21 @@ static std::vector<Gate> random_circuit(int n, int count, int seed, bool channel
     };
     for (int i = 0; i < count; i++) {
         Gate g;
-        const int r = (int)(rng() % 100);
-        if (channels && r < 3) {  // a one-qubit channel: its records are not read here
+        if (channels && rng() % 100 < 3) {  // a one-qubit channel: its records are not read here
             g.op = dmx::OP_KRAUS;
             pick(1, g.qb);
-        } else if (r < 45) {
-            g.op = dmx::OP_U3;
-            pick(1, g.qb);
-        } else if (r < 80) {
-            g.op = dmx::OP_CX;
-            pick(2, g.qb);
-        } else if (r < 88) {
-            g.op = dmx::OP_CCX;
-            pick(3, g.qb);
-        } else if (r < 92) {
-            g.op = dmx::OP_ID;
-            pick(1, g.qb);
-        } else {  // dense matrix on 2..min(5, n) qubits
-            g.op = dmx::OP_MAT;
-            g.k = 2 + (int)(rng() % (uint32_t)(std::min(5, n) - 1));
-            pick(g.k, g.qb);
-            const int d = 1 << g.k;
-            g.mre.assign((size_t)d * d, 0.0);
-            g.mim.assign((size_t)d * d, 0.0);
-            for (int j = 0; j < d; j++) g.mre[(size_t)j * d + j] = 1.0;
+            c.push_back(g);
+            continue;
         }
-        g.theta = 0.1 * i;
+        g.op = kOps[rng() % kNumOps];
+        g.theta = ang(rng), g.phi = ang(rng), g.lambda = ang(rng);
+        if (g.op == dmx::OP_MAT) {
+            g.k = 1 + (int)(rng() % (uint32_t)std::min(5, n));
+            pick(g.k, g.qb);
+            random_matrix(rng, 1 << g.k, rng() % 2 == 0, g.mre, g.mim);
+        } else {
+            pick(dmx::op_arity(g.op), g.qb);
+        }
         c.push_back(g);
     }
     return c;
@@ -70,6 +99,36 @@ static uint32_t mask_of(const Gate& g)
     uint32_t m = 0;
     for (int i = 0; i < k; i++) m |= 1u << g.qb[i];
     return m;
+}
+
+// psi <- G psi for one gate (matrix index bit k-1-j <-> qubits[j]).
+static void apply(const Gate& g, std::vector<cd>& psi)
+{
+    int k = 0;
+    std::vector<uint32_t> qs;
+    std::vector<double> re, im;
+    dmx::gate_matrix(g, k, qs, re, im);
+    const int d = 1 << k;
+    uint64_t gm = 0;
+    for (uint32_t q : qs) gm |= 1ull << q;
+    std::vector<cd> in(d), out(d);
+    std::vector<uint64_t> idx(d);
+    for (uint64_t base = 0; base < psi.size(); base++) {
+        if (base & gm) continue;
+        for (int l = 0; l < d; l++) {
+            uint64_t x = base;
+            for (int j = 0; j < k; j++)
+                if ((l >> (k - 1 - j)) & 1) x |= 1ull << qs[j];
+            idx[l] = x;
+            in[l] = psi[x];
+        }
+        for (int r = 0; r < d; r++) {
+            cd s = 0.0;
+            for (int l = 0; l < d; l++) s += cd(re[(size_t)r * d + l], im[(size_t)r * d + l]) * in[l];
+            out[r] = s;
+        }
+        for (int l = 0; l < d; l++) psi[idx[l]] = out[l];
+    }
 }
 
 int main()
@@ -100,7 +159,8 @@ int main()
                         return fail("block width", n, max_k, seed);
                     if (five && max_k < 5) {  // only a five-qubit gate makes one
                         bool wide = false;
-                        for (int gi : bp.members5[o.second]) wide |= c[gi].k == 5;
+                        for (int gi : bp.members5[o.second])
+                            wide |= __builtin_popcount(mask_of(c[gi])) == 5;
                         if (!wide) return fail("five-qubit block without a five-qubit gate",
                                                n, max_k, seed);
                     }
@@ -116,8 +176,7 @@ int main()
                 for (size_t gi = 0; gi < c.size(); gi++)
                     if (seen[gi] != (c[gi].op == dmx::OP_ID ? 0 : 1))
                         return fail("gate placed other than once", n, max_k, seed);
-                // per-qubit program order, and no gate on either side of a channel moves
-                std::vector<int> last(n, -1);
+                // no gate on either side of a channel moves across it
                 int last_channel = -1, max_before = -1;
                 for (int gi : seq) {
                     if (c[gi].op == dmx::OP_KRAUS) {
@@ -128,12 +187,24 @@ int main()
                         return fail("gate moved before a channel", n, max_k, seed);
                     }
                     max_before = std::max(max_before, gi);
-                    for (uint32_t b = mask_of(c[gi]); b; b &= b - 1) {
-                        const int q = __builtin_ctz(b);
-                        if (gi < last[q]) return fail("per-qubit order", n, max_k, seed);
-                        last[q] = gi;
-                    }
                 }
+                // the same operator in both orders (channels left out: no gate crosses one)
+                std::mt19937 rng(7u + (uint32_t)cases);
+                std::normal_distribution<double> g(0.0, 1.0);
+                std::vector<cd> a((size_t)1 << n);
+                for (cd& v : a) v = cd(g(rng), g(rng));
+                std::vector<cd> b = a;
+                for (size_t gi = 0; gi < c.size(); gi++)
+                    if (c[gi].op != dmx::OP_KRAUS && c[gi].op != dmx::OP_ID) apply(c[gi], a);
+                for (int gi : seq)
+                    if (c[gi].op != dmx::OP_KRAUS) apply(c[gi], b);
+                double err = 0, nrm = 0;
+                for (size_t i = 0; i < a.size(); i++) {
+                    err = std::max(err, std::abs(a[i] - b[i]));
+                    nrm = std::max(nrm, std::abs(a[i]));
+                }
+                if (!(err <= 1e-10 * nrm)) return fail("execution order changes the operator",
+                                                       n, max_k, seed);
                 cases++;
             }
     std::printf("%d cases OK\n", cases);
