@@ -384,6 +384,7 @@ private:
         const auto key = std::make_tuple((int)g.op, g.theta, g.phi, g.lambda);
         auto it = class_cache_.find(key);
         if (it == class_cache_.end()) {
+            if (class_cache_.size() >= kClassCacheMax) class_cache_.clear();  // distinct angles
             // classes with operand j on qubit j, then kept per operand position
             Gate p = g;
             const int k = op_arity(g.op);
@@ -561,6 +562,7 @@ private:
     static constexpr int kWindow = 64;
     static constexpr int kMaxExploredLong = 32;  // segments above kLookMaxGates
     static constexpr size_t kLookMaxGates = 1u << 16;
+    static constexpr size_t kClassCacheMax = 4096;
     const std::vector<Gate>& circuit_;
     const int n_, kmax_;
     std::vector<std::vector<int>> lists_;  // per qubit: its gates in program order
