@@ -59,6 +59,59 @@ def test_fused_random_vs_oracle(oracle, n, max_k):
     check_close(s, re, im, n)
 
 
+def commuting_heavy(n, count, seed):
+    """Gates that often commute, so the planner reorders them: CXs sharing a
+    control or a target (controls and targets drawn from small sets), CZ, CU1,
+    RZZ, RZ, U1, T, S (Z-diagonal), RX, X, RXX (X-diagonal), a few CCX, and now
+    and then an H or a U3 that commutes with neither."""
+    rng = np.random.default_rng(seed)
+    t = W.GateTable(n)
+    ctrl, tgt = [0, 1, min(2, n - 1)], [n - 1, n - 2, max(n - 3, 3)]
+    ang = lambda: float(rng.uniform(-3.1, 3.1))
+    for _ in range(count):
+        r = rng.uniform()
+        if r < 0.25:
+            c = int(rng.choice(ctrl))
+            q = int(rng.choice([x for x in range(n) if x != c]))
+            t.add("CX", [c, q])
+        elif r < 0.40:
+            q = int(rng.choice(tgt))
+            c = int(rng.choice([x for x in range(n) if x != q]))
+            t.add("CX", [c, q])
+        elif r < 0.55:
+            name = ["CZ", "CU1", "RZZ"][int(rng.integers(3))]
+            a, b = (int(x) for x in rng.choice(n, 2, replace=False))
+            t.add(name, [a, b], [] if name == "CZ" else [ang()])
+        elif r < 0.70:
+            name = ["RZ", "U1", "T", "S"][int(rng.integers(4))]
+            t.add(name, [int(rng.integers(n))], [] if name in ("T", "S") else [ang()])
+        elif r < 0.82:
+            name = ["RX", "X"][int(rng.integers(2))]
+            t.add(name, [int(rng.integers(n))], [ang()] if name == "RX" else [])
+        elif r < 0.88:
+            a, b = (int(x) for x in rng.choice(n, 2, replace=False))
+            t.add("RXX", [a, b], [ang()])
+        elif r < 0.92:
+            t.add("CCX", [int(x) for x in rng.choice(n, 3, replace=False)])
+        elif r < 0.96:
+            t.add("H", [int(rng.integers(n))])
+        else:
+            t.add("U3", [int(rng.integers(n))], [ang(), ang(), ang()])
+    return t
+
+
+@pytest.mark.parametrize("n", [6, 9])
+@pytest.mark.parametrize("max_k", [4, 5])
+def test_fused_commuting_heavy_vs_oracle(oracle, n, max_k):
+    """The planner reorders commuting gates inside its blocks; on circuits rich
+    in them the fused result still matches the oracle's program-order run."""
+    t = commuting_heavy(n, 400, 7000 + 10 * n + max_k)
+    s = run(n, [t], max_k=max_k)
+    assert 0 < s.last_run_stats()["launches"] < len(t)
+    re, im = oracle.sim(n, t.arrays())
+    check_close(s, re, im, n)
+
+
 @pytest.mark.parametrize("n,m", [(6, 4), (6, 5), (8, 4), (8, 6), (8, 7), (9, 5), (9, 8)])
 def test_fused_tile_widths_vs_oracle(oracle, n, m):
     """Every super-tile width the planner may choose; m = 4 applies only to
