@@ -285,6 +285,16 @@ int dmx_set_fusion_max_k(dmx_sim* sim, uint32_t k)
     return guarded([&] { return sim->eng.set_fusion_max_k(k); });
 }
 
+int dmx_set_fusion_permute(dmx_sim* sim, uint32_t on)
+{
+    if (!sim) return null_handle();
+    if (on > 1) {
+        dmx::set_error("fusion permute flag must be 0 or 1");
+        return DMX_EINVAL;
+    }
+    return guarded([&] { return sim->eng.set_fusion_permute(on != 0); });
+}
+
 int dmx_last_run_stats(const dmx_sim* sim, double* ms, uint32_t* launches, uint32_t* sweeps)
 {
     if (!sim) return null_handle();

@@ -38,6 +38,8 @@ Engine::~Engine()
     delete plan_;
     if (d_re_) (void)hipFree(d_re_);
     if (d_im_) (void)hipFree(d_im_);
+    if (d_re2_) (void)hipFree(d_re2_);
+    if (d_im2_) (void)hipFree(d_im2_);
     if (d_diag_) (void)hipFree(d_diag_);
     if (d_ops_) (void)hipFree(d_ops_);
     if (d_probe_blk_) (void)hipFree(d_probe_blk_);
@@ -150,9 +152,28 @@ int Engine::upload()
     }
     if (method_ == DMX_METHOD_FUSED) {
         // no plan (SRN in the circuit, or n < 4): the run uses the plain sweeps
-        int rc = build_fused_plan(circuit_, n_, fusion_max_k_, model_, tile_bits_, &plan_,
+        // below kPermuteMinQubits a sweep is latency-bound, and the index map's
+        // set-up costs more than the sweeps it saves
+        int rc = build_fused_plan(circuit_, n_, fusion_max_k_, model_, tile_bits_,
+                                  fusion_permute_ && n_ >= kPermuteMinQubits, &plan_,
                                   &plan_info_);
         if (rc) return rc;
+        if (plan_ && plan_->needs_alt() && !d_re2_) {
+            // permuting blocks run out of place: a second pair of planes, or
+            // (no room for it) a plan without them
+            hipError_t e = hipMalloc(&d_re2_, num_ * sizeof(double));
+            if (e == hipSuccess) e = hipMalloc(&d_im2_, num_ * sizeof(double));
+            if (e != hipSuccess) {
+                (void)hipGetLastError();
+                if (d_re2_) (void)hipFree(d_re2_);
+                d_re2_ = d_im2_ = nullptr;
+                delete plan_;
+                plan_ = nullptr;
+                rc = build_fused_plan(circuit_, n_, fusion_max_k_, model_, tile_bits_, false,
+                                      &plan_, &plan_info_);
+                if (rc) return rc;
+            }
+        }
         if (plan_) {
             rc = plan_->upload(stream_);
             if (rc) return rc;
@@ -247,8 +268,8 @@ int Engine::run_fused()
     last_sweeps_ = 0;
     prof_.start(stream_);
     HIPTRY(hipEventRecord(ev0_, stream_));
-    int rc = plan_->execute(d_re_, d_im_, n_, transposed_, stream_, prof_, &last_launches_,
-                            &last_sweeps_);
+    int rc = plan_->execute(d_re_, d_im_, d_re2_, d_im2_, n_, transposed_, stream_, prof_,
+                            &last_launches_, &last_sweeps_);
     if (rc) return rc;
     HIPTRY(hipEventRecord(ev1_, stream_));
     HIPTRY(hipEventSynchronize(ev1_));

@@ -33,6 +33,13 @@ public:
     int expect_z(double* out);
     int set_method(int m);
     int set_fusion_max_k(uint32_t k);
+    // Whether fused blocks may take trailing X/CX/SWAP gates as an output
+    // permutation (out-of-place sweeps); takes effect at the next upload().
+    int set_fusion_permute(bool on)
+    {
+        fusion_permute_ = on;
+        return 0;
+    }
     // Cost model for the fused planner (not owned; nullptr = greedy plan).
     // Takes effect at the next upload().
     void set_cost_model(CostModel* model) { model_ = model; }
@@ -67,8 +74,10 @@ private:
     int n_ = 0;
     uint64_t dim_ = 0;
     uint64_t num_ = 0;  // entries per plane, 4^n
-    double* d_re_ = nullptr;
-    double* d_im_ = nullptr;
+    double* d_re_ = nullptr;  // rho (the planes swap with the second pair after an
+    double* d_im_ = nullptr;  // out-of-place sweep)
+    double* d_re2_ = nullptr;  // second pair, allocated for plans with permuting blocks
+    double* d_im2_ = nullptr;
     double* d_diag_ = nullptr;
     PrimOp* d_ops_ = nullptr;
     size_t d_ops_cap_ = 0;
@@ -79,6 +88,8 @@ private:
     bool uploaded_ = false;
     int method_ = 0;
     int fusion_max_k_ = 4;
+    bool fusion_permute_ = true;
+    static constexpr int kPermuteMinQubits = 12;
     int persistent_blocks_ = 0;
     int max_blocks_ = 8192;  // grid cap for one-primitive launches
 

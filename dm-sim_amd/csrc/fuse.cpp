@@ -86,6 +86,13 @@ struct Plan : FusedPlan {
         return DMX_OK;
     }
 
+    bool needs_alt() const override
+    {
+        for (const Step& st : steps)
+            if (st.type == ST_FUSED4 && blocks[st.index].perm_on) return true;
+        return false;
+    }
+
     int upload(hipStream_t stream) override
     {
         int rc = put(&d_blocks, blocks, stream);
@@ -93,14 +100,27 @@ struct Plan : FusedPlan {
         return rc ? rc : put(&d_prims, prims, stream);
     }
 
-    int execute(double* re, double* im, int nq, bool transposed, hipStream_t stream,
-                LaunchProfiler& prof, uint32_t* launches, uint32_t* sweeps) override
+    int execute(double*& re, double*& im, double*& re2, double*& im2, int nq, bool transposed,
+                hipStream_t stream, LaunchProfiler& prof, uint32_t* launches,
+                uint32_t* sweeps) override
     {
         const double sweep_bytes = 32.0 * (double)(1ull << (2 * nq));
         uint32_t la = 0;
         for (const Step& st : steps) {
             hipError_t e = hipSuccess;
-            if (st.type == ST_FUSED4) {
+            if (st.type == ST_FUSED4 && blocks[st.index].perm_on) {
+                if (!re2 || !im2) {
+                    set_error("internal: a permuting block needs the second pair of planes");
+                    return DMX_EINVAL;
+                }
+                prof.begin(KC_FUSED4, sweep_bytes);
+                e = launch_fused4_perm(re, im, re2, im2, d_blocks + st.index, nq,
+                                       (int)blocks[st.index].m, max_blocks, stream);
+                prof.end();
+                std::swap(re, re2);
+                std::swap(im, im2);
+                la++;
+            } else if (st.type == ST_FUSED4) {
                 prof.begin(KC_FUSED4, sweep_bytes);
                 e = launch_fused4(re, im, d_blocks + st.index, nq, (int)blocks[st.index].m,
                                   max_blocks, stream);
@@ -188,8 +208,10 @@ void apply_embedded(std::vector<cd>& U, uint32_t qmask, const std::vector<uint32
 // A block under construction: qubit mask and member gates (circuit indices,
 // in application order).
 struct PB {
-    uint32_t mask;
+    uint32_t mask;            // qubits of the block's matrix gates
     std::vector<int> gates;
+    int dense = -1;           // gates[0, dense) form the matrix, the rest are a trailing
+                              // permutation; -1: all of them
 };
 using Seq = std::vector<std::pair<int, int>>;  // (0, index into blocks) or (3, channel gate)
 
@@ -286,12 +308,21 @@ void commute_classes(const Gate& g, uint32_t& zm, uint32_t& xm)
 // for at most kWindow places past the first ungrouped gate of each qubit.
 class FrontierGrouper {
 public:
-    FrontierGrouper(const std::vector<Gate>& circuit, int n, int kmax)
+    FrontierGrouper(const std::vector<Gate>& circuit, int n, int kmax, bool allow_perm)
         : circuit_(circuit), n_(n), kmax_(kmax), lists_(n), head_(n, 0),
           mask_(circuit.size(), 0), zm_(circuit.size(), 0), xm_(circuit.size(), 0),
           pos0_(circuit.size(), 0), done_(circuit.size(), 0), mark_(circuit.size(), 0),
           listed_(circuit.size(), 0), base_(n), work_(n)
     {
+        // X, CX and SWAP join as index maps only where their own matrices are
+        // exactly those permutations (operand 0 = the matrix index's high bit)
+        if (allow_perm && n <= 16) {
+            perm_op_[OP_X] = is_exact_permutation(OP_X, [](int l) { return l ^ 1; });
+            perm_op_[OP_CX] = is_exact_permutation(OP_CX, [](int l) { return l & 2 ? l ^ 1 : l; });
+            perm_op_[OP_SWAP] = is_exact_permutation(OP_SWAP, [](int l) {
+                return ((l & 1) << 1) | ((l >> 1) & 1);
+            });
+        }
     }
 
     // Groups circuit indices seg (no channels, no identities) into pbs/seq.
@@ -342,11 +373,12 @@ public:
                 }
             }
             PB pb{0, best->taken};
-            for (int gi : pb.gates) {
-                pb.mask |= mask_[gi];
-                done_[gi] = 1;
-            }
-            for (uint32_t b = pb.mask; b; b &= b - 1) {
+            for (int gi : pb.gates) pb.mask |= mask_[gi];
+            uint32_t touched = pb.mask;
+            if (perm_op_[OP_X] || perm_op_[OP_CX] || perm_op_[OP_SWAP])
+                absorb_permutations(pb, touched);
+            for (int gi : pb.gates) done_[gi] = 1;
+            for (uint32_t b = touched; b; b &= b - 1) {
                 const int q = __builtin_ctz(b);
                 while (head_[q] < (int)lists_[q].size() && done_[lists_[q][head_[q]]]) head_[q]++;
             }
@@ -404,6 +436,75 @@ private:
     bool taken(int gi) const { return done_[gi] || mark_[gi] == stamp_; }
 
     long weight(int gi) const { return __builtin_popcount(mask_[gi]) > 1 ? 100 : 1; }
+
+    // Whether op's matrix (operands on qubits 0, 1) is exactly the permutation
+    // local index l -> f(l).
+    template <class F>
+    static bool is_exact_permutation(int op, F f)
+    {
+        Gate g;
+        g.op = op;
+        for (int j = 0; j < 5; j++) g.qb[j] = (uint32_t)j;
+        int k = 0;
+        std::vector<uint32_t> qs;
+        std::vector<double> re, im;
+        gate_matrix(g, k, qs, re, im);
+        const int D = 1 << k;
+        for (int r = 0; r < D; r++)
+            for (int c = 0; c < D; c++)
+                if (re[(size_t)r * D + c] != (f(c) == r ? 1.0 : 0.0) || im[(size_t)r * D + c] != 0.0)
+                    return false;
+        return true;
+    }
+
+    // A permutation gate joins a block's index map only if the map keeps the
+    // sweep's stores whole: bit 0 stays in place (column pairs), and no
+    // condition bit inside a contiguous column run of the block's super-tile
+    // (bits below r) moves a bit outside it.
+    bool keeps_runs(int gi, int r) const
+    {
+        const Gate& g = circuit_[gi];
+        const int a = (int)g.qb[0], b = (int)g.qb[1];
+        switch (g.op) {
+        case OP_X: return true;
+        case OP_CX: return a != 0 && !(a < r && b >= r);
+        case OP_SWAP: return a != 0 && b != 0 && ((a < r) == (b < r));
+        default: return false;
+        }
+    }
+
+    // Appends to a four-qubit block the permutation gates that are ready
+    // after it (repeatedly), as its trailing index map; touched gains their
+    // qubits.
+    void absorb_permutations(PB& pb, uint32_t& touched)
+    {
+        uint32_t qm = pb.mask;
+        for (int q = 0; q < n_ && __builtin_popcount(qm) < 4; q++) qm |= 1u << q;
+        if (__builtin_popcount(qm) != 4) return;
+        uint32_t cm = qm;  // super-tile column bits at the default width
+        for (int q = 0; q < n_ && __builtin_popcount(cm) < default_tile_bits(n_); q++)
+            cm |= 1u << q;
+        const int r = __builtin_ctz(~cm);
+        const size_t dense = pb.gates.size();
+        const uint32_t all = n_ >= 32 ? ~0u : (1u << n_) - 1;
+        ++stamp_;
+        for (int gi : pb.gates) mark_[gi] = stamp_;
+        for (bool grew = true; grew;) {
+            grew = false;
+            for (int q = 0; q < n_; q++) prefix(q, work_[q]);
+            std::vector<int> ready;
+            ready_on(all, ready);
+            for (int h : ready) {
+                const int op = circuit_[h].op;
+                if (op >= OP_COUNT || !perm_op_[op] || !keeps_runs(h, r)) continue;
+                mark_[h] = stamp_;
+                pb.gates.push_back(h);
+                touched |= mask_[h];
+                grew = true;
+            }
+        }
+        if (pb.gates.size() > dense) pb.dense = (int)dense;
+    }
 
     int window_end(int q) const
     {
@@ -577,12 +678,13 @@ private:
     // classes per operand position (bit j = operand j) of the gates seen, by
     // op and angles (matrix gates are not cached)
     std::map<std::tuple<int, double, double, double>, std::pair<uint32_t, uint32_t>> class_cache_;
+    bool perm_op_[OP_COUNT] = {};  // ops that may join a block's index map
 };
 
-void group_frontier(const std::vector<Gate>& circuit, int n, int kmax, std::vector<PB>& pbs,
-                    Seq& seq)
+void group_frontier(const std::vector<Gate>& circuit, int n, int kmax, bool allow_perm,
+                    std::vector<PB>& pbs, Seq& seq)
 {
-    FrontierGrouper fg(circuit, n, kmax);
+    FrontierGrouper fg(circuit, n, kmax, allow_perm);
     std::vector<int> seg;
     for (int gi = 0; gi < (int)circuit.size(); gi++) {
         const Gate& g = circuit[gi];
@@ -609,7 +711,8 @@ double grouping_cost(const std::vector<PB>& pbs)
 
 }  // namespace
 
-int plan_blocks(const std::vector<Gate>& circuit, int n, int max_k, BlockPlan& out, bool& fusable)
+int plan_blocks(const std::vector<Gate>& circuit, int n, int max_k, BlockPlan& out, bool& fusable,
+                bool allow_perm)
 {
     out = BlockPlan{};
     fusable = false;
@@ -622,7 +725,7 @@ int plan_blocks(const std::vector<Gate>& circuit, int n, int max_k, BlockPlan& o
     std::vector<PB> pbs, pbs2;
     Seq seq, seq2;
     group_least_growth(circuit, kmax, pbs, seq);
-    group_frontier(circuit, n, kmax, pbs2, seq2);
+    group_frontier(circuit, n, kmax, allow_perm, pbs2, seq2);
     if (grouping_cost(pbs2) < grouping_cost(pbs)) {
         pbs.swap(pbs2);
         seq.swap(seq2);
@@ -634,9 +737,11 @@ int plan_blocks(const std::vector<Gate>& circuit, int n, int max_k, BlockPlan& o
         uint32_t qmask = pb.mask;
         for (int q = 0; q < n && __builtin_popcount(qmask) < 4; q++) qmask |= 1u << q;
         const int D = 1 << __builtin_popcount(qmask);
+        const int dense = pb.dense < 0 ? (int)pb.gates.size() : pb.dense;
         std::vector<cd> U((size_t)D * D, 0.0);
         for (int i = 0; i < D; i++) U[(size_t)i * D + i] = 1.0;
-        for (int gi : pb.gates) {
+        for (int t = 0; t < dense; t++) {
+            const int gi = pb.gates[t];
             int k = 0;
             std::vector<uint32_t> gq;
             std::vector<double> gre, gim;
@@ -652,9 +757,35 @@ int plan_blocks(const std::vector<Gate>& circuit, int n, int max_k, BlockPlan& o
                 fb.ur[i] = U[i].real();
                 fb.ui[i] = U[i].imag();
             }
+            if (dense < (int)pb.gates.size()) {
+                // trailing permutation pi(x) = L x + b: b = pi(0), column q = pi(2^q) + b
+                auto pi = [&](uint32_t x) {
+                    for (size_t t = dense; t < pb.gates.size(); t++) {
+                        const Gate& g = circuit[pb.gates[t]];
+                        const uint32_t a = g.qb[0], c = g.qb[1];
+                        if (g.op == OP_X) {
+                            x ^= 1u << a;
+                        } else if (g.op == OP_CX) {
+                            x ^= ((x >> a) & 1u) << c;
+                        } else {  // SWAP
+                            const uint32_t d = ((x >> a) ^ (x >> c)) & 1u;
+                            x ^= (d << a) | (d << c);
+                        }
+                    }
+                    return x;
+                };
+                fb.perm_on = 1;
+                fb.perm_b = pi(0);
+                for (int q = 0; q < n; q++) fb.perm_L[q] = pi(1u << q) ^ fb.perm_b;
+                if (fb.perm_L[0] != 1u) {  // the planner keeps bit 0; the kernel relies on it
+                    set_error("internal: a block's permutation moves index bit 0");
+                    return DMX_EINVAL;
+                }
+            }
             where[bi] = {0, (int)out.blocks.size()};
             out.blocks.push_back(fb);
             out.members.push_back(pb.gates);
+            out.dense.push_back(dense);
         } else {
             FusedBlock5 fb;
             std::memset(&fb, 0, sizeof(fb));
@@ -827,13 +958,13 @@ int choose_fused5(const std::vector<Gate>& circuit, int n, BlockPlan& bp, CostMo
 }  // namespace
 
 int build_fused_plan(const std::vector<Gate>& circuit, int n, int max_k, CostModel* model,
-                     int tile_bits, FusedPlan** out, PlanInfo* info)
+                     int tile_bits, bool allow_perm, FusedPlan** out, PlanInfo* info)
 {
     *out = nullptr;
     PlanInfo pi;
     BlockPlan bp;
     bool fusable = false;
-    int rc = plan_blocks(circuit, n, max_k, bp, fusable);
+    int rc = plan_blocks(circuit, n, max_k, bp, fusable, allow_perm);
     if (rc || !fusable) return rc;
     std::vector<std::vector<PrimOp>> plain(bp.blocks.size());
     std::vector<SubPlan> sub(bp.blocks5.size());
@@ -895,7 +1026,7 @@ int build_fused_plan(const std::vector<Gate>& circuit, int n, int max_k, CostMod
         // faster by the margin.
         FusedPlan* alt = nullptr;
         PlanInfo ai;
-        rc = build_fused_plan(circuit, n, 4, model, 0, &alt, &ai);
+        rc = build_fused_plan(circuit, n, 4, model, 0, allow_perm, &alt, &ai);
         if (rc) {
             delete plan;
             return rc;

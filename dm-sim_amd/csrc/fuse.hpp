@@ -16,11 +16,17 @@ namespace dmx {
 struct FusedPlan {
     virtual ~FusedPlan() = default;
     virtual int upload(hipStream_t stream) = 0;
+    // Whether execute needs a second pair of planes (blocks with a trailing
+    // permutation run out of place).
+    virtual bool needs_alt() const = 0;
     // Applies the whole circuit once on `stream`, timing launches into prof.
     // transposed: rho is held conj-transposed (after a plain run), which
-    // selects the channel superoperators' records.
-    virtual int execute(double* re, double* im, int n, bool transposed, hipStream_t stream,
-                        LaunchProfiler& prof, uint32_t* launches, uint32_t* sweeps) = 0;
+    // selects the channel superoperators' records.  rho is in (re, im); an
+    // out-of-place sweep writes (re2, im2) and swaps the pairs, so on return
+    // (re, im) holds rho again.
+    virtual int execute(double*& re, double*& im, double*& re2, double*& im2, int n,
+                        bool transposed, hipStream_t stream, LaunchProfiler& prof,
+                        uint32_t* launches, uint32_t* sweeps) = 0;
 };
 
 // Blocks of a circuit in execution order: order[i] = (0, index into blocks)
@@ -30,6 +36,9 @@ struct FusedPlan {
 struct BlockPlan {
     std::vector<FusedBlock> blocks;
     std::vector<std::vector<int>> members;  // circuit indices of each block's gates, in order
+    // members[b][0, dense[b]) form the block matrix; the rest are trailing
+    // permutation gates (X, CX, SWAP) applied as the block's index map
+    std::vector<int> dense;
     std::vector<FusedBlock5> blocks5;
     std::vector<std::vector<int>> members5;
     std::vector<std::pair<int, int>> order;
@@ -38,7 +47,11 @@ struct BlockPlan {
 // Greedy planning into blocks of at most max(max_k, widest gate) qubits
 // (max_k <= 5; blocks below four qubits are padded to four).  fusable = false
 // (and no blocks) when the circuit must run the plain method (SRN, n < 4).
-int plan_blocks(const std::vector<Gate>& circuit, int n, int max_k, BlockPlan& out, bool& fusable);
+// allow_perm (n <= 16): a four-qubit block may also take the X, CX and SWAP
+// gates that become ready after it, run as a permutation of its output
+// (launch_fused4_perm, out of place).
+int plan_blocks(const std::vector<Gate>& circuit, int n, int max_k, BlockPlan& out, bool& fusable,
+                bool allow_perm = false);
 
 class CostModel;  // mlp.hpp
 
@@ -66,6 +79,6 @@ struct PlanInfo {
 // tile_bits > 0 fixes every fused block's tile width where that width is
 // admissible for the block (tile_bits_ok), and takes precedence over the model.
 int build_fused_plan(const std::vector<Gate>& circuit, int n, int max_k, CostModel* model,
-                     int tile_bits, FusedPlan** out, PlanInfo* info);
+                     int tile_bits, bool allow_perm, FusedPlan** out, PlanInfo* info);
 
 }  // namespace dmx

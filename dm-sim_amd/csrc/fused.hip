@@ -56,10 +56,13 @@ __device__ __forceinline__ uint64_t deposit(uint64_t v, uint64_t mask)
 // Super-tiles st_first, st_first + st_stride, ... of one block sweep; s_re and
 // s_im are the workgroup's 16 x kPitch LDS planes.  M = number of super-tile
 // column bits (8 for n >= 8, n below that).
-template <int M>
-__device__ __forceinline__ void fused_sweep(double* __restrict__ re, double* __restrict__ im,
-                                            const FusedBlock* __restrict__ blk, int n,
-                                            uint64_t st_first, uint64_t st_stride,
+// PERM: the block has a trailing permutation pi; super-tiles are read from
+// (re, im) and written, entry (i, j) to (pi(i), pi(j)), into (re_out, im_out).
+// Otherwise re_out/im_out are re/im (in place).
+template <int M, bool PERM = false>
+__device__ __forceinline__ void fused_sweep(double* re, double* im, double* re_out,
+                                            double* im_out, const FusedBlock* __restrict__ blk,
+                                            int n, uint64_t st_first, uint64_t st_stride,
                                             double* __restrict__ s_re, double* __restrict__ s_im)
 {
     const int tid = threadIdx.x;
@@ -132,6 +135,29 @@ __device__ __forceinline__ void fused_sweep(double* __restrict__ re, double* __r
         return deposit(rg, row_free) * dim + deposit(cg, col_free) + coloff;
     };
 
+    // pi's linear part: L x = XOR of the columns of L for the set bits of x
+    // (columns loaded once; n <= 16)
+    uint32_t L[PERM ? 16 : 1];
+    if (PERM) {
+#pragma unroll
+        for (int q = 0; q < (PERM ? 16 : 1); q++) L[q] = blk->perm_L[q];
+    }
+    auto lin = [&](uint32_t x) {
+        uint32_t y = 0;
+#pragma unroll
+        for (int q = 0; q < (PERM ? 16 : 1); q++) y ^= ((x >> q) & 1u) ? L[q] : 0u;
+        return y;
+    };
+    // this thread's fixed row and column offsets inside a super-tile, mapped
+    // by L (pi(base ^ off) = L base ^ L off ^ b)
+    uint32_t l_row[PERM ? iters : 1], l_col = 0;
+    if (PERM) {
+#pragma unroll
+        for (int it = 0; it < (PERM ? iters : 1); it++)
+            l_row[it] = lin((uint32_t)deposit((uint64_t)((2 * tid + 512 * it) >> m), qmask));
+        l_col = lin((uint32_t)coloff);
+    }
+
     for (uint64_t st = st_first; st < n_st; st += st_stride) {
         const uint64_t g0 = base_of(st);
         if (active) {
@@ -191,7 +217,7 @@ __device__ __forceinline__ void fused_sweep(double* __restrict__ re, double* __r
         }
         __syncthreads();
 
-        if (active) {
+        if (active && !PERM) {
 #pragma unroll
             for (int it = 0; it < iters; it++) {
                 const int a = (2 * tid + 512 * it) >> m;
@@ -199,6 +225,25 @@ __device__ __forceinline__ void fused_sweep(double* __restrict__ re, double* __r
                     make_double2(s_re[a * kPitch + lc], s_re[a * kPitch + lc + 1]);
                 *reinterpret_cast<double2*>(im + g0 + rowoff[it]) =
                     make_double2(s_im[a * kPitch + lc], s_im[a * kPitch + lc + 1]);
+            }
+        } else if (active) {
+            const uint64_t rg = st >> cg_bits;
+            const uint64_t cg = st & ((1ull << cg_bits) - 1);
+            const uint32_t pr = lin((uint32_t)deposit(rg, row_free)) ^ blk->perm_b;
+            // pi(column pair) = (c, c ^ 1) as pi keeps bit 0: store in order
+            const uint32_t pc = lin((uint32_t)deposit(cg, col_free)) ^ blk->perm_b ^ l_col;
+            const int sw = (int)(pc & 1u);
+            const uint64_t dcol = pc & ~1u;
+#pragma unroll
+            for (int it = 0; it < iters; it++) {
+                const int a = (2 * tid + 512 * it) >> m;
+                const uint64_t off = (uint64_t)(pr ^ l_row[it]) * dim + dcol;
+                const double r0 = s_re[a * kPitch + lc], r1 = s_re[a * kPitch + lc + 1];
+                const double i0 = s_im[a * kPitch + lc], i1 = s_im[a * kPitch + lc + 1];
+                *reinterpret_cast<double2*>(re_out + off) =
+                    sw ? make_double2(r1, r0) : make_double2(r0, r1);
+                *reinterpret_cast<double2*>(im_out + off) =
+                    sw ? make_double2(i1, i0) : make_double2(i0, i1);
             }
         }
         __syncthreads();
@@ -211,7 +256,20 @@ __global__ void __launch_bounds__(256, 2) k_fused4(double* __restrict__ re, doub
 {
     __shared__ double s_re[16 * kPitch];
     __shared__ double s_im[16 * kPitch];
-    fused_sweep<M>(re, im, blk, n, blockIdx.x, gridDim.x, s_re, s_im);
+    fused_sweep<M>(re, im, re, im, blk, n, blockIdx.x, gridDim.x, s_re, s_im);
+}
+
+template <int M>
+__global__ void __launch_bounds__(256, 2) k_fused4_perm(const double* __restrict__ re,
+                                                         const double* __restrict__ im,
+                                                         double* __restrict__ re_out,
+                                                         double* __restrict__ im_out,
+                                                         const FusedBlock* __restrict__ blk, int n)
+{
+    __shared__ double s_re[16 * kPitch];
+    __shared__ double s_im[16 * kPitch];
+    fused_sweep<M, true>(const_cast<double*>(re), const_cast<double*>(im), re_out, im_out, blk,
+                         n, blockIdx.x, gridDim.x, s_re, s_im);
 }
 
 // Batched form: blockIdx.y selects one circuit's (rho, block) pair, all of
@@ -223,7 +281,7 @@ __global__ void __launch_bounds__(256, 2) k_fused4_batch(const BatchEntry* __res
     __shared__ double s_re[16 * kPitch];
     __shared__ double s_im[16 * kPitch];
     const BatchEntry e = entries[blockIdx.y];
-    fused_sweep<M>(e.re, e.im, e.blk, n, blockIdx.x, gridDim.x, s_re, s_im);
+    fused_sweep<M>(e.re, e.im, e.re, e.im, e.blk, n, blockIdx.x, gridDim.x, s_re, s_im);
 }
 
 namespace {
@@ -244,6 +302,24 @@ hipError_t launch_fused4(double* re, double* im, const FusedBlock* d_blk, int n,
     case 6: k_fused4<6><<<blocks, 256, 0, stream>>>(re, im, d_blk, n); break;
     case 7: k_fused4<7><<<blocks, 256, 0, stream>>>(re, im, d_blk, n); break;
     default: k_fused4<8><<<blocks, 256, 0, stream>>>(re, im, d_blk, n); break;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_fused4_perm(const double* re, const double* im, double* re_out, double* im_out,
+                              const FusedBlock* d_blk, int n, int m, int max_blocks,
+                              hipStream_t stream)
+{
+    if (n < 4 || n > 16 || m < 4 || m > 8 || m > n || re == re_out || im == im_out)
+        return hipErrorInvalidValue;
+    const uint64_t n_st = supertiles(n, m);
+    const unsigned blocks = (unsigned)(n_st < (uint64_t)max_blocks ? n_st : (uint64_t)max_blocks);
+    switch (m) {
+    case 4: k_fused4_perm<4><<<blocks, 256, 0, stream>>>(re, im, re_out, im_out, d_blk, n); break;
+    case 5: k_fused4_perm<5><<<blocks, 256, 0, stream>>>(re, im, re_out, im_out, d_blk, n); break;
+    case 6: k_fused4_perm<6><<<blocks, 256, 0, stream>>>(re, im, re_out, im_out, d_blk, n); break;
+    case 7: k_fused4_perm<7><<<blocks, 256, 0, stream>>>(re, im, re_out, im_out, d_blk, n); break;
+    default: k_fused4_perm<8><<<blocks, 256, 0, stream>>>(re, im, re_out, im_out, d_blk, n); break;
     }
     return hipGetLastError();
 }

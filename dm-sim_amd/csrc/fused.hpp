@@ -7,13 +7,19 @@
 namespace dmx {
 
 // A dense 4-qubit block U (16 x 16, row-major, local index bit t <-> the t-th
-// lowest qubit of qmask).
+// lowest qubit of qmask), optionally followed by a permutation of the basis
+// (trailing X, CX and SWAP gates): when perm_on, entry (i, j) of U rho U^dagger
+// goes to (pi(i), pi(j)) with pi(x) = L x + b over GF(2), L given by its
+// columns perm_L[q] = the image of bit q (q < n), and the sweep writes into a
+// second pair of planes.  pi leaves bit 0 in place (perm_L[0] = 1).
 struct FusedBlock {
     uint32_t qmask;  // exactly four qubits
     uint32_t m;      // super-tile column bits (the tile schedule), see tile_bits_ok
-    uint32_t pad[2];
+    uint32_t perm_on;
+    uint32_t perm_b;
     double ur[256];
     double ui[256];
+    uint32_t perm_L[16];
 };
 
 // One circuit's rho and the block to apply to it (batched launches).
@@ -61,6 +67,13 @@ hipError_t launch_fused5(double* re, double* im, const FusedBlock5* d_blk, int n
 // 16 rows x 2^m columns.
 hipError_t launch_fused4(double* re, double* im, const FusedBlock* d_blk, int n, int m,
                          int max_blocks, hipStream_t stream);
+
+// The same for a block with a trailing permutation (perm_on; n <= 16): reads
+// rho from (re, im), writes the permuted result to (re_out, im_out), which
+// must not overlap the input.
+hipError_t launch_fused4_perm(const double* re, const double* im, double* re_out, double* im_out,
+                              const FusedBlock* d_blk, int n, int m, int max_blocks,
+                              hipStream_t stream);
 
 // The same for `count` circuits of n qubits each (one launch of about
 // max_blocks workgroups in all).

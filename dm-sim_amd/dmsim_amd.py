@@ -28,7 +28,7 @@ EXPORTS = [
     "dmx_clear_circuit", "dmx_reset", "dmx_reset_dm", "dmx_measure", "dmx_get_dm",
     "dmx_get_entries",
     "dmx_get_diag", "dmx_expect_z", "dmx_info", "dmx_dump", "dmx_set_method",
-    "dmx_set_fusion_max_k", "dmx_last_run_stats", "dmx_set_profiling", "dmx_profile_get",
+    "dmx_set_fusion_max_k", "dmx_set_fusion_permute", "dmx_last_run_stats", "dmx_set_profiling", "dmx_profile_get",
     "dmx_qasm_parse", "dmx_append_qasm",
     "dmx_cost_create", "dmx_cost_destroy", "dmx_cost_features", "dmx_cost_train",
     "dmx_cost_predict", "dmx_cost_gradient", "dmx_cost_get_params", "dmx_cost_set_params",
@@ -76,6 +76,7 @@ def load():
     lib.dmx_dump.argtypes = [vp, C.c_char_p, C.c_size_t, P(C.c_size_t)]
     lib.dmx_set_method.argtypes = [vp, C.c_int]
     lib.dmx_set_fusion_max_k.argtypes = [vp, C.c_uint32]
+    lib.dmx_set_fusion_permute.argtypes = [vp, C.c_uint32]
     lib.dmx_last_run_stats.argtypes = [vp, P(C.c_double), P(C.c_uint32), P(C.c_uint32)]
     lib.dmx_set_profiling.argtypes = [vp, C.c_int]
     lib.dmx_profile_get.argtypes = [vp, C.c_uint32, C.c_char_p, C.c_size_t, P(C.c_uint32),
@@ -261,6 +262,11 @@ class Sim:
 
     def set_fusion_max_k(self, k: int):
         _check(load().dmx_set_fusion_max_k(self._h, k))
+
+    def set_fusion_permute(self, on: bool):
+        """Let fused blocks take trailing X/CX/SWAP gates as an output
+        permutation (default on; needs a second copy of rho)."""
+        _check(load().dmx_set_fusion_permute(self._h, 1 if on else 0))
 
     def set_profiling(self, on: bool):
         _check(load().dmx_set_profiling(self._h, 1 if on else 0))

@@ -143,6 +143,15 @@ int dmx_set_method(dmx_sim* sim, int method);
  * default 4); a gate on five qubits always forms a five-qubit block. */
 int dmx_set_fusion_max_k(dmx_sim* sim, uint32_t k);
 
+/* Extension with no counterpart in the reference: whether four-qubit blocks of
+ * DMX_METHOD_FUSED may also take the X, CX and SWAP gates that follow them as
+ * a permutation of their output (on = 1, default; applied from 12 qubits up,
+ * where a sweep is bandwidth-bound).  Such sweeps run out of place and need a
+ * second pair of planes (16*4^n bytes, allocated at the first upload that uses
+ * them, or the plan does without); off = 0 keeps every sweep in place.  Used
+ * from the next dmx_upload. */
+int dmx_set_fusion_permute(dmx_sim* sim, uint32_t on);
+
 /* Device time of the last run() in milliseconds (HIP events on the run's
  * stream), number of kernel launches it made, and number of full-matrix
  * sweeps it performed. */

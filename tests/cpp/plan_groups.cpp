@@ -1,10 +1,13 @@
 // Validity of the fused planner's grouping (dmx::plan_blocks) on seeded random
-// circuits, checked on the host: every gate lands in exactly one block, every
-// block holds its members' qubits within its width, no gate crosses a
-// channel, and the blocks' execution order (which may swap commuting gates)
-// applies the same operator as program order -- both orders are applied to a
-// random state with the gates' own matrices (dmx::gate_matrix) and compared.
-// Prints "<cases> cases OK", or the first violation and exits nonzero.
+// circuits, with and without trailing permutations, checked on the host:
+// every gate lands in exactly one block, every block holds its matrix
+// members' qubits within its width, its trailing members are X, CX or SWAP
+// and its stored index map (perm_L, perm_b) is what they do to an index, no
+// gate crosses a channel, and the blocks' execution order (which may swap
+// commuting gates) applies the same operator as program order -- both orders
+// are applied to a random state with the gates' own matrices
+// (dmx::gate_matrix) and compared.  Prints "<cases> cases OK", or the first
+// violation and exits nonzero.
 #include <algorithm>
 #include <cmath>
 #include <complex>
@@ -23,6 +26,25 @@ static int fail(const char* what, int n, int k, int seed)
 {
     std::printf("FAIL %s (n=%d max_k=%d seed=%d)\n", what, n, k, seed);
     return 1;
+}
+
+// What a block's trailing permutation gates do to index x.
+static uint32_t permute(const std::vector<Gate>& c, const std::vector<int>& mem, int from,
+                        uint32_t x)
+{
+    for (size_t t = from; t < mem.size(); t++) {
+        const Gate& g = c[mem[t]];
+        const uint32_t a = g.qb[0], b = g.qb[1];
+        if (g.op == dmx::OP_X) {
+            x ^= 1u << a;
+        } else if (g.op == dmx::OP_CX) {
+            if ((x >> a) & 1u) x ^= 1u << b;
+        } else {
+            const uint32_t ba = (x >> a) & 1u, bb = (x >> b) & 1u;
+            x = (x & ~((1u << a) | (1u << b))) | (ba << b) | (bb << a);
+        }
+    }
+    return x;
 }
 
 // Random d x d unitary (Gram-Schmidt on Gaussian columns), or a random
@@ -134,6 +156,7 @@ static void apply(const Gate& g, std::vector<cd>& psi)
 int main()
 {
     int cases = 0;
+    for (int perm = 0; perm < 2; perm++)
     for (int n = 4; n <= 12; n++)
         for (int max_k = 4; max_k <= 5; max_k++)
             for (int seed = 0; seed < 6; seed++) {
@@ -142,8 +165,9 @@ int main()
                                                            channels);
                 dmx::BlockPlan bp;
                 bool fusable = false;
-                if (dmx::plan_blocks(c, n, max_k, bp, fusable) != 0 || !fusable)
+                if (dmx::plan_blocks(c, n, max_k, bp, fusable, perm == 1) != 0 || !fusable)
                     return fail("plan_blocks", n, max_k, seed);
+                if (bp.dense.size() != bp.blocks.size()) return fail("dense counts", n, max_k, seed);
                 // execution sequence of circuit indices (channels included)
                 std::vector<int> seq;
                 for (const auto& o : bp.order) {
@@ -165,10 +189,32 @@ int main()
                                                n, max_k, seed);
                     }
                     const auto& mem = five ? bp.members5[o.second] : bp.members[o.second];
-                    for (int gi : mem) {
-                        if ((mask_of(c[gi]) & ~qmask) != 0)
+                    const int dense = five ? (int)mem.size() : bp.dense[o.second];
+                    for (int t = 0; t < (int)mem.size(); t++) {
+                        const int gi = mem[t];
+                        const int op = c[gi].op;
+                        if (t < dense && (mask_of(c[gi]) & ~qmask) != 0)
                             return fail("member outside its block", n, max_k, seed);
+                        if (t >= dense && op != dmx::OP_X && op != dmx::OP_CX && op != dmx::OP_SWAP)
+                            return fail("trailing member not a permutation gate", n, max_k, seed);
                         seq.push_back(gi);
+                    }
+                    if (!five) {
+                        const dmx::FusedBlock& fb = bp.blocks[o.second];
+                        if ((fb.perm_on != 0) != (dense < (int)mem.size()) || (perm == 0 && fb.perm_on))
+                            return fail("permutation flag", n, max_k, seed);
+                        if (fb.perm_on) {
+                            if (fb.perm_L[0] != 1u) return fail("permutation moves bit 0", n, max_k, seed);
+                            std::mt19937 r2((uint32_t)(seed + 31 * n));
+                            for (int trial = 0; trial < 64; trial++) {
+                                const uint32_t x = (uint32_t)r2() & ((1u << n) - 1);
+                                uint32_t y = fb.perm_b;
+                                for (int q = 0; q < n; q++)
+                                    if ((x >> q) & 1u) y ^= fb.perm_L[q];
+                                if (y != permute(c, mem, dense, x))
+                                    return fail("stored index map", n, max_k, seed);
+                            }
+                        }
                     }
                 }
                 std::vector<int> seen(c.size(), 0);

@@ -120,8 +120,10 @@ def test_planner_grouping_is_valid(tmp_path):
     """The fused planner's grouping (dmx::plan_blocks, an exported C++ symbol of
     the library) on 108 seeded random circuits of 4..12 qubits with one- to
     five-qubit gates, identities and channels, blocks of at most four and five
-    qubits: every gate in exactly one block, per-qubit program order kept, no
-    gate across a channel, members within their block (tests/cpp/plan_groups.cpp).
+    qubits, with and without trailing permutations: every gate in exactly one
+    block, matrix members within their block, trailing index maps as stored, no
+    gate across a channel, and execution order equal to program order as an
+    operator on a random state (tests/cpp/plan_groups.cpp).
     The internal headers include HIP's, so hipcc compiles it (host code only)."""
     import shutil
     import subprocess
@@ -136,4 +138,4 @@ def test_planner_grouping_is_valid(tmp_path):
     r = subprocess.run(cmd, capture_output=True, text=True)
     assert r.returncode == 0, r.stderr
     r = subprocess.run([exe], capture_output=True, text=True, timeout=120)
-    assert r.returncode == 0 and r.stdout.strip() == "108 cases OK", r.stdout
+    assert r.returncode == 0 and r.stdout.strip() == "216 cases OK", r.stdout

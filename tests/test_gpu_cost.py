@@ -177,12 +177,14 @@ def test_five_qubit_launch_timing(dm):
 
 
 @pytest.mark.parametrize("n", [8, 10])
-def test_learned_plan_with_five_qubit_blocks(dm, fitted, n):
-    """Blocks of up to five qubits: the learned plan, which may split a
-    five-qubit block into four-qubit ones, against the greedy plan and unfused
-    execution."""
+def test_learned_plan_with_five_qubit_blocks(dm, costdata, n):
+    """Blocks of up to five qubits: the learned plan with the shipped model
+    (fitted to this GPU's launches at 4..16 qubits), which may split a
+    five-qubit block into four-qubit ones or take the four-qubit plan, against
+    the greedy plan and unfused execution."""
     import workloads as W
-    model = fitted[0]
+    model = costdata.default_model()
+    assert model is not None
     table = W.random_mixed(n, 300, seed=200 + n)
     print(f"cost: five-qubit learned plan vs greedy vs plain, n={n}", flush=True)
 
