@@ -151,6 +151,7 @@ public:
     }
     void set_method(int m) { check(dmx_set_method(h_, m)); }
     void set_fusion_max_k(uint32_t k) { check(dmx_set_fusion_max_k(h_, k)); }
+    void set_fusion_permute(bool on) { check(dmx_set_fusion_permute(h_, on ? 1u : 0u)); }
     // extension: the gates of an OpenQASM 2.0 program (tool/dmsim_qasm.py's job)
     uint32_t append_qasm(const std::string& source)
     {
@@ -232,6 +233,7 @@ PYBIND11_MODULE(libdmsim_py_amdgpu_omp, m)
         .def("dump", &Simulation::dump)
         .def("set_method", &Simulation::set_method)
         .def("set_fusion_max_k", &Simulation::set_fusion_max_k)
+        .def("set_fusion_permute", &Simulation::set_fusion_permute)
         .def("append_qasm", &Simulation::append_qasm)
         .def("last_run_stats", &Simulation::last_run_stats)
         .def_property_readonly("n_qubits", &Simulation::n_qubits)
