@@ -353,30 +353,42 @@ public:
             search({}, cap, c1);
             std::sort(c1.begin(), c1.end(), ranked);
             const Cand* best = &c1[0];
-            if (look > 0 && c1[0].taken.size() < left) {
+            const bool perms = perm_op_[OP_X] || perm_op_[OP_CX] || perm_op_[OP_SWAP];
+            // a candidate with the permutation gates it would take after it
+            auto with_perms = [&](const Cand& c, PB& pb, uint32_t& touched) {
+                pb = PB{0, c.taken};
+                for (int gi : pb.gates) pb.mask |= mask_[gi];
+                touched = pb.mask;
+                if (perms) absorb_permutations(pb, touched);
+            };
+            PB pb;
+            uint32_t touched = 0;
+            with_perms(*best, pb, touched);
+            if (look > 0 && pb.gates.size() < left) {
                 long best_total = -1;
                 for (int i = 0; i < (int)c1.size() && i < look; i++) {
+                    PB cand;
+                    uint32_t t = 0;
+                    with_perms(c1[i], cand, t);
                     long total = c1[i].score;
-                    if (c1[i].taken.size() == left) {
+                    for (size_t k = c1[i].taken.size(); k < cand.gates.size(); k++)
+                        total += weight(cand.gates[k]);
+                    if (cand.gates.size() == left) {
                         total += 1L << 40;  // finishes the segment
                     } else {
                         std::vector<Cand> c2;
-                        search(c1[i].taken, kMaxExplored2, c2);
+                        search(cand.gates, kMaxExplored2, c2);
                         long s2 = 0;
                         for (const Cand& c : c2) s2 = std::max(s2, c.score);
                         total += s2;
                     }
                     if (total > best_total) {
                         best_total = total;
-                        best = &c1[i];
+                        pb = std::move(cand);
+                        touched = t;
                     }
                 }
             }
-            PB pb{0, best->taken};
-            for (int gi : pb.gates) pb.mask |= mask_[gi];
-            uint32_t touched = pb.mask;
-            if (perm_op_[OP_X] || perm_op_[OP_CX] || perm_op_[OP_SWAP])
-                absorb_permutations(pb, touched);
             for (int gi : pb.gates) done_[gi] = 1;
             for (uint32_t b = touched; b; b &= b - 1) {
                 const int q = __builtin_ctz(b);
