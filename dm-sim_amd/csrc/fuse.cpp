@@ -300,12 +300,16 @@ void commute_classes(const Gate& g, uint32_t& zm, uint32_t& xm)
 // further ready gate's qubits while it stays within kmax.  The absorption
 // order is an order of the circuit's gates that differs from program order
 // only by swaps of commuting gates, so the blocks' product is the circuit's.
-// Every Q reached this way (depth-first, memoised, at most kMaxExplored per
+// Every Q reached this way (best first, memoised, at most kMaxExplored per
 // search) is a candidate, scored by the gates it absorbs: 100 per multi-qubit
-// gate, 1 per one-qubit gate (those join almost any block later).  The kLook
-// best candidates are compared by their score plus the best score of the
-// block that would follow them; the smaller Q wins ties.  A gate is looked
-// for at most kWindow places past the first ungrouped gate of each qubit.
+// gate, 1 per one-qubit gate (those join almost any block later).  With
+// allow_perm, a chosen four-qubit block also takes the X, CX and SWAP gates
+// that become ready after it, as the permutation of its output (they follow
+// all of its matrix gates, so the order is still a valid one).  The kLook best
+// candidates, each with the permutation gates it would take, are compared by
+// their score plus the best score of the block that would follow them; the
+// smaller Q wins ties.  A gate is looked for at most kWindow places past the
+// first ungrouped gate of each qubit.
 class FrontierGrouper {
 public:
     FrontierGrouper(const std::vector<Gate>& circuit, int n, int kmax, bool allow_perm)
