@@ -152,3 +152,52 @@ def test_channels_mixed_methods_continuation():
         s.run()
         rho = apply_dense(n, prog, rho)
     assert np.max(np.abs(got_rho(s, n) - rho)) <= 1e-10
+
+
+def test_channels_between_permuting_sweeps():
+    """12 qubits, CX/SWAP/X-rich segments separated by channels: the fused plan
+    with out-of-place permuting sweeps (the planes swap after each) keeps the
+    channels in place and matches the in-place fused plan and the plain
+    method, over two consecutive runs (the second starts from the plain
+    method's conj-transposed orientation for the plain simulation)."""
+    dm = dmx()
+    n = 12
+    rng = np.random.default_rng(1212)
+    chunks = []
+    for c in range(6):
+        t = W.GateTable(n)
+        for _ in range(40):
+            r = rng.uniform()
+            if r < 0.5:
+                t.add("CX", [int(x) for x in rng.choice(n, 2, replace=False)])
+            elif r < 0.6:
+                t.add("SWAP", [int(x) for x in rng.choice(n, 2, replace=False)])
+            elif r < 0.7:
+                t.add("X", [int(rng.integers(n))])
+            else:
+                t.add("U3", [int(rng.integers(n))], [float(v) for v in rng.uniform(0, 6.2, 3)])
+        ops = dm.depolarizing(0.1) if c % 2 else dm.amplitude_damping(0.2)
+        chunks.append((t, int(rng.integers(n)), ops))
+
+    def make(method, permute):
+        s = dm.Sim(n)
+        s.set_method(method)
+        s.set_fusion_permute(permute)
+        for t, q, ops in chunks:
+            s.append_table(t)
+            s.append_kraus(q, ops)
+        s.upload()
+        s.run()
+        s.run()
+        return s
+
+    plain = make(1, False)
+    ref = got_rho(plain, n)
+    launches = {}
+    for permute in (False, True):
+        s = make(2, permute)
+        got = got_rho(s, n)
+        assert np.max(np.abs(got - ref)) <= 1e-10, permute
+        assert abs(np.trace(got) - 1) <= 1e-12
+        launches[permute] = s.last_run_stats()["launches"]
+    assert launches[True] < launches[False]
