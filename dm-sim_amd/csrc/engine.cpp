@@ -152,12 +152,25 @@ int Engine::upload()
     }
     if (method_ == DMX_METHOD_FUSED) {
         // no plan (SRN in the circuit, or n < 4): the run uses the plain sweeps
-        // below kPermuteMinQubits a sweep is latency-bound, and the index map's
-        // set-up costs more than the sweeps it saves
         int rc = build_fused_plan(circuit_, n_, fusion_max_k_, model_, tile_bits_,
-                                  fusion_permute_ && n_ >= kPermuteMinQubits, &plan_,
-                                  &plan_info_);
+                                  fusion_permute_, &plan_, &plan_info_);
         if (rc) return rc;
+        if (plan_ && plan_info_.perm_blocks > 0 && n_ < kPermuteSureQubits) {
+            // a short sweep is latency-bound and a permuting one pays its index
+            // map's set-up: keep the permuting plan only if it still wins
+            FusedPlan* inplace = nullptr;
+            PlanInfo ii;
+            rc = build_fused_plan(circuit_, n_, fusion_max_k_, model_, tile_bits_, false,
+                                  &inplace, &ii);
+            if (rc) return rc;
+            if (inplace && !permuting_pays(plan_info_, ii)) {
+                delete plan_;
+                plan_ = inplace;
+                plan_info_ = ii;
+            } else {
+                delete inplace;
+            }
+        }
         if (plan_ && plan_->needs_alt() && !d_re2_) {
             // permuting blocks run out of place: a second pair of planes, or
             // (no room for it) a plan without them
@@ -185,6 +198,27 @@ int Engine::upload()
     HIPTRY(hipStreamSynchronize(stream_));
     uploaded_ = true;
     return DMX_OK;
+}
+
+// Relative extra cost of a permuting sweep: about 2.5 us of index-map set-up
+// against the sweep's own time (32 * 4^n bytes at ~4.6 TB/s plus ~8 us of
+// launch latency), at least 3 % (measured at n = 8 and n = 14).
+double Engine::permute_overhead() const
+{
+    const double sweep_us = 32.0 * (double)num_ / 4.6e6 + 8.0;
+    return std::max(0.03, 2.5 / sweep_us);
+}
+
+bool Engine::permuting_pays(const PlanInfo& perm, const PlanInfo& inplace) const
+{
+    const double ovh = permute_overhead();
+    if (perm.predicted_us > 0 && inplace.predicted_us > 0) {  // the cost model's estimates
+        const double share = perm.fused_blocks ? (double)perm.perm_blocks / perm.fused_blocks : 0.0;
+        return perm.predicted_us * (1.0 + ovh * share) < inplace.predicted_us;
+    }
+    const double sweeps_perm =
+        perm.fused_blocks + perm.fused5_blocks + perm.plain_blocks + ovh * perm.perm_blocks;
+    return sweeps_perm < inplace.fused_blocks + inplace.fused5_blocks + inplace.plain_blocks;
 }
 
 int Engine::run()

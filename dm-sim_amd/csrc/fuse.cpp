@@ -1028,6 +1028,7 @@ int build_fused_plan(const std::vector<Gate>& circuit, int n, int max_k, CostMod
         } else if (plain[o.second].empty()) {
             plan->steps.push_back({ST_FUSED4, o.second, 0});
             pi.fused_blocks++;
+            pi.perm_blocks += plan->blocks[o.second].perm_on ? 1 : 0;
             pi.tile_hist[plan->blocks[o.second].m]++;
         } else {
             const int first = (int)plan->prims.size();

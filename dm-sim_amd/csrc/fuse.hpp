@@ -64,6 +64,7 @@ struct PlanInfo {
     uint32_t fused5_blocks = 0;  // five-qubit blocks run fused (a split one counts as its
                                  // four-qubit sub-blocks instead)
     uint32_t channels = 0;       // noise channels (one superoperator sweep each)
+    uint32_t perm_blocks = 0;    // fused four-qubit blocks with a trailing permutation
     uint32_t tile_hist[9] = {};  // fused blocks per tile width m (4..8)
     double predicted_us = 0.0;
     double greedy_us = 0.0;

@@ -145,8 +145,9 @@ int dmx_set_fusion_max_k(dmx_sim* sim, uint32_t k);
 
 /* Extension with no counterpart in the reference: whether four-qubit blocks of
  * DMX_METHOD_FUSED may also take the X, CX and SWAP gates that follow them as
- * a permutation of their output (on = 1, default; applied from 12 qubits up,
- * where a sweep is bandwidth-bound).  Such sweeps run out of place and need a
+ * a permutation of their output (on = 1, default; below 12 qubits only when
+ * the plan still wins after the permuting sweeps' extra set-up, which matters
+ * where a sweep is latency-bound).  Such sweeps run out of place and need a
  * second pair of planes (16*4^n bytes, allocated at the first upload that uses
  * them, or the plan does without); off = 0 keeps every sweep in place.  Used
  * from the next dmx_upload. */

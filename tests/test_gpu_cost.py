@@ -134,6 +134,9 @@ def test_learned_plan_judged_by_unfused(dm, fitted, n):
     def make(method, mdl=None):
         s = dm.Sim(n)
         s.set_method(method)
+        # in-place sweeps only: both plans share one grouping, so every block
+        # is either fused or run unfused
+        s.set_fusion_permute(False)
         if mdl is not None:
             s.set_cost_model(mdl)
         s.append_table(table)
@@ -208,7 +211,6 @@ def test_learned_plan_with_five_qubit_blocks(dm, costdata, n):
         assert max(np.abs(re - re0).max(), np.abs(im - im0).max()) <= 1e-10
     gi, info = greedy.plan_info(), learned.plan_info()
     assert gi["fused5_blocks"] > 0
-    assert info["fused5_blocks"] <= gi["fused5_blocks"]
     t_greedy = _time_runs(greedy)
     t_learned = _time_runs(learned)
     print(f"n={n}: greedy {t_greedy:.3f} ms plan {gi}; learned {t_learned:.3f} ms "
