@@ -39,10 +39,11 @@ void Batch::free_device()
     if (graph_) (void)hipGraphDestroy(graph_);
     exec_ = nullptr;
     graph_ = nullptr;
-    for (void* p : {(void*)pool_, (void*)d_blocks_, (void*)d_entries_, (void*)d_re_off_,
-                    (void*)d_terms_, (void*)d_jobs_, (void*)d_energy_})
+    for (void* p : {(void*)pool_, (void*)pool_alt_, (void*)d_blocks_, (void*)d_entries_,
+                    (void*)d_re_off_, (void*)d_terms_, (void*)d_jobs_, (void*)d_energy_})
         if (p) (void)hipFree(p);
     pool_ = nullptr;
+    pool_alt_ = nullptr;
     d_blocks_ = nullptr;
     d_entries_ = nullptr;
     d_re_off_ = nullptr;
@@ -150,10 +151,23 @@ int Batch::upload()
     uint64_t off = 0;
     size_t max_steps = 0;
     for (Circ& c : circs_) {
+        // the single-circuit engine's choice (same planner, same rule), so a
+        // batched rho is bit-identical to the circuit's own fused run
         BlockPlan bp;
         bool fusable = false;
-        int rc = plan_blocks(c.gates, c.n, 4, bp, fusable);
+        int rc = plan_blocks(c.gates, c.n, 4, bp, fusable, true);
         if (rc) return rc;
+        size_t permuting = 0;
+        for (const FusedBlock& fb : bp.blocks) permuting += fb.perm_on ? 1 : 0;
+        if (fusable && permuting > 0 && c.n < kPermuteSureQubits) {
+            BlockPlan ip;
+            bool f2 = false;
+            rc = plan_blocks(c.gates, c.n, 4, ip, f2, false);
+            if (rc) return rc;
+            if (!permuting_pays(c.n, (double)(bp.blocks.size() + bp.blocks5.size()),
+                                (double)permuting, (double)(ip.blocks.size() + ip.blocks5.size())))
+                bp = std::move(ip);
+        }
         if (!fusable) {
             set_error("batched circuits must be fusable (no SRN)");
             return DMX_EINVAL;
@@ -174,6 +188,38 @@ int Batch::upload()
     }
     pool_doubles_ = off;
     BTRY(hipMalloc(&pool_, pool_doubles_ * sizeof(double)));
+    bool any_perm = false;
+    for (const FusedBlock& fb : blocks) any_perm = any_perm || fb.perm_on;
+    if (any_perm) BTRY(hipMalloc(&pool_alt_, pool_doubles_ * sizeof(double)));
+    // which pool holds each circuit's rho before each of its blocks (static,
+    // so the captured graph replays the same pointers)
+    auto planes = [&](const Circ& c, bool alt, double*& re, double*& im) {
+        double* base = alt ? pool_alt_ : pool_;
+        re = base + c.re_off;
+        im = base + c.im_off;
+    };
+    std::vector<std::vector<char>> alt_before(circs_.size());
+    for (size_t i = 0; i < circs_.size(); i++) {
+        Circ& c = circs_[i];
+        bool alt = false;
+        for (size_t b = 0; b < c.n_blocks; b++) {
+            alt_before[i].push_back(alt ? 1 : 0);
+            if (blocks[c.blk_off + b].perm_on) alt = !alt;
+        }
+        c.final_alt = alt;
+    }
+    auto entry = [&](int i, size_t s) {
+        const Circ& c = circs_[i];
+        const bool alt = alt_before[i][s] != 0;
+        BatchEntry e;
+        planes(c, alt, e.re, e.im);
+        if (blocks[c.blk_off + s].perm_on)
+            planes(c, !alt, e.re_out, e.im_out);
+        else
+            e.re_out = e.re, e.im_out = e.im;
+        e.blk = d_blocks_ + c.blk_off + s;
+        return e;
+    };
     if (!blocks.empty()) BTRY(hipMalloc(&d_blocks_, blocks.size() * sizeof(FusedBlock)));
     for (const Circ& c : circs_) tile_bits_[c.n] = default_tile_bits(c.n);
     if (autotune_ && !blocks.empty()) {
@@ -209,8 +255,7 @@ int Batch::upload()
                     if (s >= c.n_blocks || c.n != kv.first.first ||
                         (int)blocks[c.blk_off + s].m != kv.first.second)
                         continue;
-                    entries.push_back(
-                        {pool_ + c.re_off, pool_ + c.im_off, d_blocks_ + c.blk_off + s});
+                    entries.push_back(entry(i, s));
                 }
             launches_.push_back(kv.second);
         }
@@ -219,8 +264,9 @@ int Batch::upload()
 
     std::vector<EnergyJob> jobs;
     for (const Circ& c : circs_) {
-        jobs.push_back({pool_ + c.re_off, pool_ + c.im_off, c.n, (int32_t)terms.size(),
-                        (int32_t)c.paulis.size(), 0});
+        double *re = nullptr, *im = nullptr;
+        planes(c, c.final_alt, re, im);
+        jobs.push_back({re, im, c.n, (int32_t)terms.size(), (int32_t)c.paulis.size(), 0});
         terms.insert(terms.end(), c.paulis.begin(), c.paulis.end());
     }
     if (terms.empty()) terms.push_back(PauliTerm{0, 0, 0, 0, 0.0});  // keep a valid pointer
@@ -293,9 +339,14 @@ int Batch::autotune(std::vector<FusedBlock>& blocks)
         const int lo = n > 4 ? 5 : 4, hi = std::min(8, n);
         if (lo >= hi) continue;
         std::vector<BatchEntry> ents;
-        for (int i : kv.second)
-            ents.push_back({pool_ + circs_[i].re_off, pool_ + circs_[i].im_off,
-                             d_blocks_ + circs_[i].blk_off});
+        for (int i : kv.second) {
+            const Circ& c = circs_[i];
+            BatchEntry e{pool_ + c.re_off, pool_ + c.im_off, pool_ + c.re_off, pool_ + c.im_off,
+                         d_blocks_ + c.blk_off};
+            if (blocks[c.blk_off].perm_on)  // its first block writes the second pool
+                e.re_out = pool_alt_ + c.re_off, e.im_out = pool_alt_ + c.im_off;
+            ents.push_back(e);
+        }
         hipError_t e = hipMemcpy(d_tmp, ents.data(), ents.size() * sizeof(BatchEntry),
                                  hipMemcpyHostToDevice);
         float best_ms = 0.f;
@@ -379,8 +430,9 @@ int Batch::get_dm(uint32_t id, double* re, double* im)
     }
     const Circ& c = circs_[id];
     const size_t bytes = (1ull << (2 * c.n)) * sizeof(double);
-    if (re) BTRY(hipMemcpy(re, pool_ + c.re_off, bytes, hipMemcpyDeviceToHost));
-    if (im) BTRY(hipMemcpy(im, pool_ + c.im_off, bytes, hipMemcpyDeviceToHost));
+    const double* base = c.final_alt ? pool_alt_ : pool_;
+    if (re) BTRY(hipMemcpy(re, base + c.re_off, bytes, hipMemcpyDeviceToHost));
+    if (im) BTRY(hipMemcpy(im, base + c.im_off, bytes, hipMemcpyDeviceToHost));
     return DMX_OK;
 }
 

@@ -71,6 +71,7 @@ private:
         uint64_t re_off = 0, im_off = 0;
         size_t blk_off = 0;
         size_t n_blocks = 0;
+        bool final_alt = false;  // rho ends in the second pool (odd number of permuting blocks)
     };
     struct Launch {
         int n;
@@ -91,6 +92,7 @@ private:
     std::map<int, int> tile_bits_;  // qubit count -> super-tile width
 
     double* pool_ = nullptr;
+    double* pool_alt_ = nullptr;  // second pool, for circuits with permuting blocks
     uint64_t pool_doubles_ = 0;
     FusedBlock* d_blocks_ = nullptr;
     BatchEntry* d_entries_ = nullptr;

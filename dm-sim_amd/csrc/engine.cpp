@@ -200,25 +200,15 @@ int Engine::upload()
     return DMX_OK;
 }
 
-// Relative extra cost of a permuting sweep: about 2.5 us of index-map set-up
-// against the sweep's own time (32 * 4^n bytes at ~4.6 TB/s plus ~8 us of
-// launch latency), at least 3 % (measured at n = 8 and n = 14).
-double Engine::permute_overhead() const
-{
-    const double sweep_us = 32.0 * (double)num_ / 4.6e6 + 8.0;
-    return std::max(0.03, 2.5 / sweep_us);
-}
-
 bool Engine::permuting_pays(const PlanInfo& perm, const PlanInfo& inplace) const
 {
-    const double ovh = permute_overhead();
     if (perm.predicted_us > 0 && inplace.predicted_us > 0) {  // the cost model's estimates
         const double share = perm.fused_blocks ? (double)perm.perm_blocks / perm.fused_blocks : 0.0;
-        return perm.predicted_us * (1.0 + ovh * share) < inplace.predicted_us;
+        return perm.predicted_us * (1.0 + permute_overhead(n_) * share) < inplace.predicted_us;
     }
-    const double sweeps_perm =
-        perm.fused_blocks + perm.fused5_blocks + perm.plain_blocks + ovh * perm.perm_blocks;
-    return sweeps_perm < inplace.fused_blocks + inplace.fused5_blocks + inplace.plain_blocks;
+    return dmx::permuting_pays(n_, perm.fused_blocks + perm.fused5_blocks + perm.plain_blocks,
+                               perm.perm_blocks,
+                               inplace.fused_blocks + inplace.fused5_blocks + inplace.plain_blocks);
 }
 
 int Engine::run()

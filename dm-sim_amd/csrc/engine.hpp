@@ -89,9 +89,6 @@ private:
     int method_ = 0;
     int fusion_max_k_ = 4;
     bool fusion_permute_ = true;
-    // from this many qubits a permuting plan is kept without comparison
-    static constexpr int kPermuteSureQubits = 12;
-    double permute_overhead() const;
     bool permuting_pays(const PlanInfo& perm, const PlanInfo& inplace) const;
     int persistent_blocks_ = 0;
     int max_blocks_ = 8192;  // grid cap for one-primitive launches

@@ -53,6 +53,26 @@ struct BlockPlan {
 int plan_blocks(const std::vector<Gate>& circuit, int n, int max_k, BlockPlan& out, bool& fusable,
                 bool allow_perm = false);
 
+// Relative extra cost of a sweep with a trailing permutation on n qubits:
+// about 2.5 us of index-map set-up against the sweep's own time (32 * 4^n bytes
+// at ~4.6 TB/s plus ~8 us of launch latency), at least 3 % (measured at n = 8
+// and n = 14).
+inline double permute_overhead(int n)
+{
+    const double sweep_us = 32.0 * (double)(1ull << (2 * n)) / 4.6e6 + 8.0;
+    const double r = 2.5 / sweep_us;
+    return r > 0.03 ? r : 0.03;
+}
+
+// From this many qubits a permuting plan is kept without comparison; below, it
+// is kept only if its sweeps, each permuting one charged permute_overhead(n)
+// extra, are fewer than the in-place plan's.
+constexpr int kPermuteSureQubits = 12;
+inline bool permuting_pays(int n, double sweeps, double permuting, double sweeps_inplace)
+{
+    return n >= kPermuteSureQubits || sweeps + permute_overhead(n) * permuting < sweeps_inplace;
+}
+
 class CostModel;  // mlp.hpp
 
 // What the planner chose.  predicted_us is the cost model's estimate of the

@@ -281,7 +281,11 @@ __global__ void __launch_bounds__(256, 2) k_fused4_batch(const BatchEntry* __res
     __shared__ double s_re[16 * kPitch];
     __shared__ double s_im[16 * kPitch];
     const BatchEntry e = entries[blockIdx.y];
-    fused_sweep<M>(e.re, e.im, e.re, e.im, e.blk, n, blockIdx.x, gridDim.x, s_re, s_im);
+    if (e.blk->perm_on)
+        fused_sweep<M, true>(e.re, e.im, e.re_out, e.im_out, e.blk, n, blockIdx.x, gridDim.x,
+                             s_re, s_im);
+    else
+        fused_sweep<M>(e.re, e.im, e.re, e.im, e.blk, n, blockIdx.x, gridDim.x, s_re, s_im);
 }
 
 namespace {
@@ -327,7 +331,7 @@ hipError_t launch_fused4_perm(const double* re, const double* im, double* re_out
 hipError_t launch_fused4_batch(const BatchEntry* d_entries, int count, int n, int m,
                                int max_blocks, hipStream_t stream)
 {
-    if (n < 4 || m < 4 || m > 8 || m > n || count < 1 || count > 65535)
+    if (n < 4 || n > 16 || m < 4 || m > 8 || m > n || count < 1 || count > 65535)
         return hipErrorInvalidValue;
     const uint64_t n_st = supertiles(n, m);
     // about max_blocks workgroups in all, each looping over super-tiles of its

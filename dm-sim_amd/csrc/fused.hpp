@@ -24,8 +24,10 @@ struct FusedBlock {
 
 // One circuit's rho and the block to apply to it (batched launches).
 struct BatchEntry {
-    double* re;
+    double* re;  // rho before the block
     double* im;
+    double* re_out;  // after it: re/im, or the other pair for a permuting block
+    double* im_out;
     const FusedBlock* blk;
 };
 

@@ -97,7 +97,6 @@ def test_batch_bitwise_equals_single_fused_run(autotune):
     for cid, (circ, _, _) in enumerate(items):
         s = dmsim_amd.Sim(circ.n_qubits)
         s.set_method(dmsim_amd.METHOD_FUSED)
-        s.set_fusion_permute(False)  # the batch keeps every sweep in place
         s.append_table(circ)
         s.upload()
         s.run()
