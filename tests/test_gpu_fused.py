@@ -114,8 +114,8 @@ def test_fused_commuting_heavy_vs_oracle(oracle, n, max_k):
 
 @pytest.mark.parametrize("n,m", [(12, 0), (12, 5), (12, 7)])
 def test_fused_trailing_permutations(oracle, n, m):
-    """Blocks that also take the X/CX/SWAP gates after them (from 12 qubits
-    up) run out of place with a permuted output; at the default and at fixed
+    """Blocks that also take the X/CX/SWAP gates after them (kept without
+    comparison from 12 qubits up) run out of place with a permuted output; at the default and at fixed
     tile widths the result matches the in-place plan and the oracle, with
     fewer sweeps, and a run after a plain continuation still starts from the
     right planes."""
