@@ -174,12 +174,16 @@ int Engine::upload()
         if (plan_ && plan_->needs_alt() && !d_re2_) {
             // permuting blocks run out of place: a second pair of planes, or
             // (no room for it) a plan without them
-            hipError_t e = hipMalloc(&d_re2_, num_ * sizeof(double));
-            if (e == hipSuccess) e = hipMalloc(&d_im2_, num_ * sizeof(double));
-            if (e != hipSuccess) {
+            double *re2 = nullptr, *im2 = nullptr;
+            hipError_t e = hipMalloc(&re2, num_ * sizeof(double));
+            const bool got_re = e == hipSuccess;
+            if (got_re) e = hipMalloc(&im2, num_ * sizeof(double));
+            if (e == hipSuccess) {
+                d_re2_ = re2;
+                d_im2_ = im2;
+            } else {
                 (void)hipGetLastError();
-                if (d_re2_) (void)hipFree(d_re2_);
-                d_re2_ = d_im2_ = nullptr;
+                if (got_re) (void)hipFree(re2);
                 delete plan_;
                 plan_ = nullptr;
                 rc = build_fused_plan(circuit_, n_, fusion_max_k_, model_, tile_bits_, false,
