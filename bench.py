@@ -29,10 +29,10 @@ sys.path.insert(0, os.path.join(ROOT, "dm-sim_amd"))
 
 METRIC = "ρ gate-updates/s at n=14 FP64: fused+batched vs plain gate-by-gate; HBM GB/s"
 HBM_PEAK_GBS = 8000.0
-# HBM bytes per launch over algorithmic bytes per launch, from rocprofv3 FETCH_SIZE and
-# WRITE_SIZE runs of this workload scaled by a copy of known size
-# (profiles/k_fused4_n14_hbm_counters.md).
-TRAFFIC_RATIO = {"k_fused4": 8.591e9 / 8.590e9}
+# HBM bytes per launch over algorithmic bytes per launch, per fused sweep kernel,
+# from rocprofv3 FETCH_SIZE and WRITE_SIZE runs of this workload scaled by a copy
+# of known size (profiles/k_fused4_n14_hbm_counters.md).
+TRAFFIC_RATIO = {"k_fused4": 8.591e9 / 8.590e9, "k_fused4_perm": 8.591e9 / 8.590e9}
 METHODS = {"plain": 0, "plain_launch": 1, "fused": 2, "fused_learned": 2, "fused_learned5": 2}
 
 

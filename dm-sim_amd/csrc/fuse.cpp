@@ -113,7 +113,7 @@ struct Plan : FusedPlan {
                     set_error("internal: a permuting block needs the second pair of planes");
                     return DMX_EINVAL;
                 }
-                prof.begin(KC_FUSED4, sweep_bytes);
+                prof.begin(KC_FUSED4P, sweep_bytes);
                 e = launch_fused4_perm(re, im, re2, im2, d_blocks + st.index, nq,
                                        (int)blocks[st.index].m, max_blocks, stream);
                 prof.end();

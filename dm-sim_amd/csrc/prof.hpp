@@ -15,6 +15,7 @@ enum KernelClass : int {
     KC_QUAD,          // k_quad, 4x4 primitive
     KC_PERSISTENT,    // k_plain_persistent, a segment of primitives
     KC_FUSED5,        // k_fused5: one dense 5-qubit block, both sides of rho
+    KC_FUSED4P,       // k_fused4_perm: a 4-qubit block with a trailing permutation, out of place
     KC_COUNT
 };
 
