@@ -68,6 +68,7 @@ public:
 private:
     int run_plain(bool per_launch);
     int run_fused();
+    void drop_graph();
     int ensure_ops_capacity(size_t count);
     int to_reference_orientation();
 
@@ -96,6 +97,19 @@ private:
     std::vector<Gate> circuit_;
     std::vector<PrimOp> prims_;
     FusedPlan* plan_ = nullptr;
+    // the fused run captured and replayed (profiling off): valid for the
+    // orientation and the planes it starts from; it records the planes it ends
+    // in (permuting sweeps swap the pairs, so a plan with an odd number of them
+    // alternates between two graphs)
+    struct RunGraph {
+        hipGraphExec_t exec = nullptr;
+        bool transposed = false;
+        double *re = nullptr, *im = nullptr;  // rho at the start
+        double *re_end = nullptr, *im_end = nullptr, *re2_end = nullptr, *im2_end = nullptr;
+        uint32_t launches = 0;
+    };
+    RunGraph graphs_[2];
+    int graph_next_ = 0;
     CostModel* model_ = nullptr;
     int tile_bits_ = 0;
     PlanInfo plan_info_;

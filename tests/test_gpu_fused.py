@@ -165,6 +165,58 @@ def test_fused_trailing_permutations(oracle, n, m):
     assert max(np.abs(a[0] - b[0]).max(), np.abs(a[1] - b[1]).max()) <= 3 * TOL
 
 
+def permutation_rich(n, count, seed):
+    rng = np.random.default_rng(seed)
+    t = W.GateTable(n)
+    for _ in range(count):
+        r = rng.uniform()
+        if r < 0.5:
+            t.add("CX", [int(x) for x in rng.choice(n, 2, replace=False)])
+        elif r < 0.6:
+            t.add("SWAP", [int(x) for x in rng.choice(n, 2, replace=False)])
+        else:
+            t.add("U3", [int(rng.integers(n))], [float(v) for v in rng.uniform(0, 6.2, 3)])
+    return t
+
+
+@pytest.mark.parametrize("n", [8, 12])
+def test_fused_graph_replays_match_eager(n):
+    """Without profiling, a fused run is captured into a HIP graph and
+    replayed (a plan with an odd number of permuting sweeps alternates between
+    two graphs, one per starting pair of planes).  Every replay -- with
+    downloads (synchronizations) between them and a plain-method run that
+    flips rho's orientation, so the channel records change and the graphs are
+    re-captured -- equals the eager run of the same plan bit for bit."""
+    dm = dmx()
+    if n == 8:
+        t = W.random_mixed(n, 200, 880, gate_set=[g for g in UNITARY_SET if W.ARITY[g] <= 4])
+    else:
+        t = permutation_rich(n, 150, 1200)
+
+    def make(profiling):
+        s = dm.Sim(n)
+        s.set_method(FUSED)
+        s.set_profiling(profiling)
+        s.append_table(t)
+        s.append_kraus(3, dm.amplitude_damping(0.2))
+        s.append_table(t)
+        s.upload()
+        return s
+
+    eager, graph = make(True), make(False)
+    for step in range(5):
+        if step == 3:  # a plain run in between: orientation flips
+            for s in (eager, graph):
+                s.set_method(1)
+                s.run()
+                s.set_method(FUSED)
+        for s in (eager, graph):
+            s.run()
+        a, b = eager.get_dm(), graph.get_dm()
+        assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1]), step
+        assert eager.last_run_stats()["launches"] == graph.last_run_stats()["launches"]
+
+
 @pytest.mark.parametrize("n,m", [(6, 4), (6, 5), (8, 4), (8, 6), (8, 7), (9, 5), (9, 8)])
 def test_fused_tile_widths_vs_oracle(oracle, n, m):
     """Every super-tile width the planner may choose; m = 4 applies only to
