@@ -2,6 +2,7 @@
 #include <cstring>
 #include <exception>
 #include <new>
+#include <set>
 #include <string>
 #include <vector>
 
@@ -12,12 +13,19 @@
 #include "mlp.hpp"
 #include "qasm.hpp"
 
+struct dmx_cost;
+
 struct dmx_sim {
     dmx::Engine eng;
+    dmx_cost* model = nullptr;  // the planner model set on this simulation
 };
 
+// A model remembers the simulations that plan with it, so destroying it
+// first detaches it from them (their later uploads, including the one a
+// method switch makes, plan greedily instead of reading freed memory).
 struct dmx_cost {
     dmx::CostModel m;
+    std::set<dmx_sim*> users;
 };
 
 static_assert(DMX_COST_FEATURES == dmx::kMlpFeatures, "feature count");
@@ -74,6 +82,7 @@ int dmx_create(uint32_t n_qubits, uint32_t n_gpus, dmx_sim** out)
 
 int dmx_destroy(dmx_sim* sim)
 {
+    if (sim && sim->model) sim->model->users.erase(sim);
     delete sim;
     return DMX_OK;
 }
@@ -304,6 +313,20 @@ int dmx_last_run_stats(const dmx_sim* sim, double* ms, uint32_t* launches, uint3
     return DMX_OK;
 }
 
+int dmx_set_report(dmx_sim* sim, int on)
+{
+    if (!sim) return null_handle();
+    sim->eng.set_report(on != 0);
+    return DMX_OK;
+}
+
+int dmx_last_run_fallbacks(const dmx_sim* sim, uint32_t* segments)
+{
+    if (!sim) return null_handle();
+    if (segments) *segments = sim->eng.last_fallbacks();
+    return DMX_OK;
+}
+
 int dmx_set_profiling(dmx_sim* sim, int on)
 {
     if (!sim) return null_handle();
@@ -404,6 +427,11 @@ int dmx_cost_create(uint32_t seed, dmx_cost** out)
 
 int dmx_cost_destroy(dmx_cost* model)
 {
+    if (model)
+        for (dmx_sim* s : model->users) {
+            s->eng.set_cost_model(nullptr);
+            s->model = nullptr;
+        }
     delete model;
     return DMX_OK;
 }
@@ -504,6 +532,9 @@ int dmx_cost_set_params(dmx_cost* model, const float* params, float mean, float 
 int dmx_set_cost_model(dmx_sim* sim, dmx_cost* model)
 {
     if (!sim) return null_handle();
+    if (sim->model) sim->model->users.erase(sim);
+    sim->model = model;
+    if (model) model->users.insert(sim);
     sim->eng.set_cost_model(model ? &model->m : nullptr);
     return DMX_OK;
 }

@@ -225,8 +225,28 @@ int Engine::run()
         set_error("run() before upload()");
         return DMX_ESTATE;
     }
-    if (method_ == DMX_METHOD_FUSED && plan_) return run_fused();
-    return run_plain(method_ != DMX_METHOD_PLAIN);
+    const int rc = method_ == DMX_METHOD_FUSED && plan_ ? run_fused()
+                                                        : run_plain(method_ != DMX_METHOD_PLAIN);
+    if (rc == DMX_OK && report_) {
+        // the reference's per-run report (src/dmsim_amdgpu_omp.hpp:498-503,
+        // on by default through PRINT_MEA_PER_CIRCUIT); one GPU: no
+        // communication time
+        const double mb = (double)device_bytes() / 1024.0 / 1024.0;
+        printf("\n============== DM-Sim ===============\n");
+        printf("nqubits:%d, ngates:%u, ngpus:%d, comp:%.3lf ms, comm:%.3lf ms, sim:%.3lf ms, "
+               "mem:%.3lf MB, mem_per_gpu:%.3lf MB\n",
+               n_, n_gates(), 1, last_ms_, 0.0, last_ms_, mb, mb);
+        printf("=====================================\n");
+        fflush(stdout);
+    }
+    return rc;
+}
+
+uint64_t Engine::device_bytes() const
+{
+    uint64_t b = 2 * num_ * sizeof(double) + dim_ * sizeof(double) + d_ops_cap_ * sizeof(PrimOp);
+    if (d_re2_) b += 2 * num_ * sizeof(double);
+    return b;
 }
 
 int Engine::run_plain(bool per_launch)
@@ -234,6 +254,8 @@ int Engine::run_plain(bool per_launch)
     const int n_ops = (int)prims_.size();
     last_launches_ = 0;
     last_sweeps_ = 0;
+    last_fallbacks_ = 0;
+    bool coop_ok = true;  // until a cooperative launch of this run is refused
     prof_.start(stream_);
     HIPTRY(hipEventRecord(ev0_, stream_));
     // Segments of gate primitives between channels: each segment runs its two
@@ -263,12 +285,33 @@ int Engine::run_plain(bool per_launch)
                     double bytes = 0;
                     while (end < s1 && prims_[end].kind != PK_C2)
                         bytes += prim_bytes(prims_[end++], n_);
-                    prof_.begin(KC_PERSISTENT, bytes);
-                    HIPTRY(launch_plain_persistent(d_re_, d_im_, d_ops_, t, end, n_, slow,
-                                                   persistent_blocks_, stream_));
-                    prof_.end();
-                    last_launches_++;
-                    t = end;
+                    hipError_t e = hipErrorCooperativeLaunchTooLarge;
+                    if (coop_ok) {
+                        prof_.begin(KC_PERSISTENT, bytes);
+                        e = launch_plain_persistent(d_re_, d_im_, d_ops_, t, end, n_, slow,
+                                                    persistent_blocks_, stream_);
+                        prof_.end();
+                    }
+                    if (e == hipSuccess) {
+                        last_launches_++;
+                        t = end;
+                        continue;
+                    }
+                    // The cooperative launch was refused (ROCm grants the
+                    // grid barrier's hardware to one process on a device at
+                    // a time) and did not run: the same primitives, one
+                    // launch each, give the same bits.
+                    (void)hipGetLastError();
+                    if (coop_ok) prof_.cancel();
+                    coop_ok = false;
+                    last_fallbacks_++;
+                    for (; t < end; t++) {
+                        prof_.begin(prim_class(prims_[t]), prim_bytes(prims_[t], n_));
+                        HIPTRY(launch_plain_one(d_re_, d_im_, d_ops_ + t, prims_[t], n_, slow,
+                                                max_blocks_, stream_));
+                        prof_.end();
+                        last_launches_++;
+                    }
                 }
             }
             last_sweeps_ += 2u * (uint32_t)(s1 - s0);
@@ -307,6 +350,7 @@ int Engine::run_fused()
 {
     last_launches_ = 0;
     last_sweeps_ = 0;
+    last_fallbacks_ = 0;
     if (!prof_.enabled()) {
         // launch-bound at small n: one graph launch instead of one per step
         RunGraph* rg = nullptr;

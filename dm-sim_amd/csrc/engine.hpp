@@ -63,6 +63,13 @@ public:
     double last_ms() const { return last_ms_; }
     uint32_t last_launches() const { return last_launches_; }
     uint32_t last_sweeps() const { return last_sweeps_; }
+    // Persistent-loop segments of the last run that ran as one launch per
+    // primitive because the cooperative launch was refused.
+    uint32_t last_fallbacks() const { return last_fallbacks_; }
+    // Prints the reference's per-run report after every run (off by default).
+    void set_report(bool on) { report_ = on; }
+    // Device memory this simulation holds (rho, second planes, tables).
+    uint64_t device_bytes() const;
     LaunchProfiler& profiler() { return prof_; }
 
 private:
@@ -121,6 +128,8 @@ private:
     double last_ms_ = 0.0;
     uint32_t last_launches_ = 0;
     uint32_t last_sweeps_ = 0;
+    uint32_t last_fallbacks_ = 0;
+    bool report_ = false;
 };
 
 // thread-local error slot for the C ABI

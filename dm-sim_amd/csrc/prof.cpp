@@ -45,6 +45,11 @@ void LaunchProfiler::end()
     if (recs_.back().b) (void)hipEventRecord(recs_.back().b, stream_);
 }
 
+void LaunchProfiler::cancel()
+{
+    if (on_ && !recs_.empty()) recs_.pop_back();
+}
+
 hipError_t LaunchProfiler::collect()
 {
     for (const Rec& r : recs_) {

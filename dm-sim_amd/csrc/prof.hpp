@@ -35,6 +35,7 @@ public:
     void start(hipStream_t s);  // new run
     void begin(int cls, double bytes);
     void end();
+    void cancel();  // drops the record of the last begin() (its launch did not run)
     hipError_t collect();  // after the stream is synchronised
     const KernelStat& stat(int cls) const { return stats_[cls]; }
 

@@ -54,7 +54,9 @@ public:
     Simulation(uint32_t n_qubits, uint32_t n_gpus)
     {
         check(dmx_create(n_qubits, n_gpus, &h_));
+        check(dmx_set_report(h_, 1));  // the reference prints after every run
     }
+    void set_report(bool on) { check(dmx_set_report(h_, on ? 1 : 0)); }
     ~Simulation() { dmx_destroy(h_); }
     Simulation(const Simulation&) = delete;
     Simulation& operator=(const Simulation&) = delete;
@@ -164,10 +166,13 @@ public:
         double ms = 0;
         uint32_t launches = 0, sweeps = 0;
         check(dmx_last_run_stats(h_, &ms, &launches, &sweeps));
+        uint32_t fallbacks = 0;
+        check(dmx_last_run_fallbacks(h_, &fallbacks));
         py::dict d;
         d["ms"] = ms;
         d["launches"] = launches;
         d["sweeps"] = sweeps;
+        d["fallbacks"] = fallbacks;
         return d;
     }
     uint32_t n_qubits()
@@ -236,6 +241,7 @@ PYBIND11_MODULE(libdmsim_py_amdgpu_omp, m)
         .def("set_fusion_permute", &Simulation::set_fusion_permute)
         .def("append_qasm", &Simulation::append_qasm)
         .def("last_run_stats", &Simulation::last_run_stats)
+        .def("set_report", &Simulation::set_report)
         .def_property_readonly("n_qubits", &Simulation::n_qubits)
         .def_property_readonly("dim", &Simulation::dim)
         .def_property_readonly("n_gates", &Simulation::n_gates)

@@ -116,6 +116,12 @@ const std::map<std::string, Builtin>& builtins()
 
 // Expanded gate count limit (nested user gates can multiply a short text).
 constexpr size_t kMaxGates = 1u << 22;
+// Gate applications, user-defined ones included: nested definitions whose
+// bodies emit nothing (empty or barrier-only) still multiply the work.
+constexpr size_t kMaxApplications = 1u << 24;
+// Nesting of expressions, unary signs and powers in one expression (the
+// recursive descent must not exhaust the host stack).
+constexpr int kMaxExprDepth = 1024;
 
 struct GateDef {
     std::vector<std::string> params;
@@ -131,6 +137,25 @@ struct Parser {
     std::map<std::string, int> cregs;
     std::map<std::string, GateDef> defs;
     QasmProgram* prog = nullptr;
+    size_t applications = 0;
+    int expr_depth = 0;
+
+    // Counts one level of expression nesting for the lifetime of the object.
+    struct Nest {
+        Parser& p;
+        Nest(Parser& parser, int line) : p(parser)
+        {
+            if (++p.expr_depth > kMaxExprDepth) {
+                p.expr_depth = 0;
+                throw QasmError(line, "expression nests more than " +
+                                          std::to_string(kMaxExprDepth) + " levels deep");
+            }
+        }
+        ~Nest()
+        {
+            if (p.expr_depth > 0) p.expr_depth--;
+        }
+    };
 
     const Tok& peek() const { return toks[pos]; }
     const Tok& next() { return toks[pos++]; }
@@ -166,6 +191,7 @@ struct Parser {
     using Env = std::map<std::string, double>;
     double expr(const Env& env)
     {
+        Nest nest(*this, peek().line);
         double v = term(env);
         while (is_sym("+") || is_sym("-")) {
             const bool add = next().s == "+";
@@ -186,6 +212,7 @@ struct Parser {
     }
     double unary(const Env& env)
     {
+        Nest nest(*this, peek().line);
         if (is_sym("-")) {
             pos++;
             return -unary(env);
@@ -254,6 +281,10 @@ struct Parser {
                const std::vector<int>& qs, int line, int depth)
     {
         if (depth > 64) throw QasmError(line, "gate definitions nest too deeply");
+        if (++applications > kMaxApplications)
+            throw QasmError(line, "the program applies more than " +
+                                      std::to_string(kMaxApplications) +
+                                      " gates, user-defined ones included");
         auto d = defs.find(name);
         if (d != defs.end()) {
             const GateDef& g = d->second;

@@ -28,7 +28,8 @@ EXPORTS = [
     "dmx_clear_circuit", "dmx_reset", "dmx_reset_dm", "dmx_measure", "dmx_get_dm",
     "dmx_get_entries",
     "dmx_get_diag", "dmx_expect_z", "dmx_info", "dmx_dump", "dmx_set_method",
-    "dmx_set_fusion_max_k", "dmx_set_fusion_permute", "dmx_last_run_stats", "dmx_set_profiling", "dmx_profile_get",
+    "dmx_set_fusion_max_k", "dmx_set_fusion_permute", "dmx_last_run_stats", "dmx_last_run_fallbacks", "dmx_set_report",
+    "dmx_set_profiling", "dmx_profile_get",
     "dmx_qasm_parse", "dmx_append_qasm",
     "dmx_cost_create", "dmx_cost_destroy", "dmx_cost_features", "dmx_cost_train",
     "dmx_cost_predict", "dmx_cost_gradient", "dmx_cost_get_params", "dmx_cost_set_params",
@@ -78,6 +79,8 @@ def load():
     lib.dmx_set_fusion_max_k.argtypes = [vp, C.c_uint32]
     lib.dmx_set_fusion_permute.argtypes = [vp, C.c_uint32]
     lib.dmx_last_run_stats.argtypes = [vp, P(C.c_double), P(C.c_uint32), P(C.c_uint32)]
+    lib.dmx_last_run_fallbacks.argtypes = [vp, P(C.c_uint32)]
+    lib.dmx_set_report.argtypes = [vp, C.c_int]
     lib.dmx_set_profiling.argtypes = [vp, C.c_int]
     lib.dmx_profile_get.argtypes = [vp, C.c_uint32, C.c_char_p, C.c_size_t, P(C.c_uint32),
                                     P(C.c_double), P(C.c_double), P(C.c_uint32)]
@@ -271,6 +274,10 @@ class Sim:
     def set_profiling(self, on: bool):
         _check(load().dmx_set_profiling(self._h, 1 if on else 0))
 
+    def set_report(self, on: bool):
+        """Print the reference's per-run report after every run (off here)."""
+        _check(load().dmx_set_report(self._h, 1 if on else 0))
+
     def profile(self):
         """{class name: {calls, ms, bytes}} of the last profiled run."""
         lib = load()
@@ -290,7 +297,12 @@ class Sim:
     def last_run_stats(self):
         ms, la, sw = C.c_double(), C.c_uint32(), C.c_uint32()
         _check(load().dmx_last_run_stats(self._h, C.byref(ms), C.byref(la), C.byref(sw)))
-        return {"ms": ms.value, "launches": la.value, "sweeps": sw.value}
+        fb = C.c_uint32()
+        _check(load().dmx_last_run_fallbacks(self._h, C.byref(fb)))
+        # fallbacks: persistent-loop segments run one launch per primitive
+        # because the cooperative launch was refused
+        return {"ms": ms.value, "launches": la.value, "sweeps": sw.value,
+                "fallbacks": fb.value}
 
     def append_qasm(self, source: str):
         """Append an OpenQASM 2.0 program's gates; returns how many."""

@@ -116,7 +116,7 @@ int dmx_measure(dmx_sim* sim, uint32_t repetition, uint32_t seed, uint32_t* out,
 int dmx_get_dm(dmx_sim* sim, double* re, double* im);
 
 /* Extension: selected entries of dm_real_res / dm_imag_res without copying
- * the whole matrix (which is 64 GiB per plane at n = 16): re[t], im[t] =
+ * the whole matrix (32 GiB per plane, 64 GiB for rho, at n = 16): re[t], im[t] =
  * entry idx[t] (row-major linear index < 4^n, same convention as dmx_get_dm).
  * re or im may be NULL. */
 int dmx_get_entries(dmx_sim* sim, uint64_t count, const uint64_t* idx, double* re, double* im);
@@ -157,6 +157,22 @@ int dmx_set_fusion_permute(dmx_sim* sim, uint32_t on);
  * stream), number of kernel launches it made, and number of full-matrix
  * sweeps it performed. */
 int dmx_last_run_stats(const dmx_sim* sim, double* ms, uint32_t* launches, uint32_t* sweeps);
+
+/* The reference's per-run report (src/dmsim_amdgpu_omp.hpp:498-503, printed
+ * after every sim() when PRINT_MEA_PER_CIRCUIT is defined, as it is by
+ * default, src/config.hpp:25): on = 1 prints the same banner and line
+ * ("nqubits:.., ngates:.., ngpus:.., comp:.. ms, comm:.. ms, sim:.. ms,
+ * mem:.. MB, mem_per_gpu:.. MB") to stdout after every dmx_run; on = 0 (the
+ * C ABI's default) is silent.  The C++ and Python faces turn it on. */
+int dmx_set_report(dmx_sim* sim, int on);
+
+/* Extension: how many persistent-loop segments of the last DMX_METHOD_PLAIN
+ * run fell back to one launch per primitive (bit-identical results) because
+ * the device refused the cooperative launch, as ROCm does while another
+ * process holds the device's grid-barrier resource.  The reference's
+ * hipLaunchCooperativeKernel call (src/dmsim_amdgpu_omp.hpp:420-433) exits
+ * the process in that case. */
+int dmx_last_run_fallbacks(const dmx_sim* sim, uint32_t* segments);
 
 /* Extension: time every launch of subsequent runs with a pair of HIP events on
  * the run's stream (on = 1), or stop (on = 0). */
@@ -246,8 +262,11 @@ int dmx_cost_gradient(dmx_cost* model, uint32_t count, const float* X, const flo
 int dmx_cost_get_params(const dmx_cost* model, float* params, float* mean, float* dev);
 int dmx_cost_set_params(dmx_cost* model, const float* params, float mean, float dev);
 
-/* Planner model of a simulation (NULL = greedy plan); used from the next
- * dmx_upload with DMX_METHOD_FUSED.  The model must outlive that upload. */
+/* Planner model of a simulation (NULL = greedy plan); used by every later
+ * dmx_upload with DMX_METHOD_FUSED, including the one dmx_set_method makes
+ * when it switches between the fused and the plain executors.  Destroying the
+ * model detaches it from every simulation it was set on (they plan greedily
+ * from then on). */
 int dmx_set_cost_model(dmx_sim* sim, dmx_cost* model);
 
 /* Fixed super-tile width m (4..8) for every fused block that admits it

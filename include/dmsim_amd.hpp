@@ -77,8 +77,11 @@ public:
           dm_imag_res(this, 1)
     {
         detail::check(dmx_create(n, g, &h_));
+        detail::check(dmx_set_report(h_, 1));  // PRINT_MEA_PER_CIRCUIT, src/config.hpp:25
     }
     ~Simulation() { dmx_destroy(h_); }
+    // Extension: turns the per-run report of sim() off (on by default).
+    void set_report(bool on) { detail::check(dmx_set_report(h_, on ? 1 : 0)); }
     Simulation(const Simulation&) = delete;
     Simulation& operator=(const Simulation&) = delete;
 

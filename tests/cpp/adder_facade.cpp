@@ -45,6 +45,9 @@ int main(int argc, char** argv)
     unmaj(sim, 0, 5, 1);
     sim.upload();
     sim.sim();
+    uint32_t fallbacks = 0;
+    detail::check(dmx_last_run_fallbacks(sim.handle(), &fallbacks));
+    printf("persistent segments run as single launches: %u\n", fallbacks);
     IdxType* res = sim.measure(5, 1234u);
     print_measurement(res, 10, 5);
     int bad = 0;

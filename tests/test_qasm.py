@@ -159,3 +159,30 @@ def test_expansion_limit(dm):
     lines.append("g30 q[0];")
     with pytest.raises(ValueError, match="expands to more than"):
         dm.qasm_parse("\n".join(lines))
+
+
+def test_application_limit_without_emitted_gates(dm):
+    """Nested definitions whose bodies emit nothing (here a barrier only)
+    still double the work at every level: the parser stops after a bounded
+    number of applications instead of running for 2^60 calls."""
+    lines = ["qreg q[1];", "gate g0 a { barrier a; }"]
+    for k in range(1, 61):
+        lines.append(f"gate g{k} a {{ g{k - 1} a; g{k - 1} a; }}")
+    lines.append("g60 q[0];")
+    with pytest.raises(ValueError, match="applies more than"):
+        dm.qasm_parse("\n".join(lines))
+
+
+@pytest.mark.parametrize("prefix,suffix", [("(" * 100000, "1" + ")" * 100000),
+                                           ("-" * 100000, "1"),
+                                           ("2^" * 100000, "1")])
+def test_expression_depth_limit(dm, prefix, suffix):
+    """Deeply nested parentheses, signs or powers end in a parse error that
+    names the line, not in a stack overflow of the host process."""
+    with pytest.raises(ValueError, match="line 2.*nests more than"):
+        dm.qasm_parse("qreg q[1];\nrx(" + prefix + suffix + ") q[0];")
+
+
+def test_expression_depth_below_limit(dm):
+    nq, gates = table(dm, "qreg q[1];\nrx(" + "(" * 200 + "0.5" + ")" * 200 + ") q[0];")
+    assert gates[0][2][0] == 0.5
