@@ -76,19 +76,29 @@ static int g_threads = 1;
 
 void orc_set_threads(int t) { g_threads = t > 0 ? t : 1; }
 
+/* Threads of a primitive's own loop: g_threads, or 1 inside the row-wise
+ * forward pass (whose rows are already spread over the threads). */
+static int team(void)
+{
+#ifdef _OPENMP
+    return omp_in_parallel() ? 1 : g_threads;
+#else
+    return 1;
+#endif
+}
+
 /* Pair enumeration within each row (src/dmsim_cpu_omp.hpp:959-968): pair p of
  * a row maps to pos0 = p with a 0 inserted at bit q, pos1 = pos0 + 2^q. */
 #define PAIR_LOOP_BEGIN(R, Q)                                                   \
     const idx_t dim_ = (idx_t)1 << (R)->n;                                      \
     const idx_t half_ = dim_ >> 1;                                              \
-    const idx_t total_ = half_ * (R)->rows;                                     \
+    const idx_t rows_ = (R)->rows;                                              \
     const idx_t lowmask_ = ((idx_t)1 << (Q)) - 1;                               \
     double* const dr = (R)->re;                                                 \
     double* const di = (R)->im;                                                 \
-    _Pragma("omp parallel for num_threads(g_threads) schedule(static)")         \
-    for (idx_t i_ = 0; i_ < total_; i_++) {                                     \
-        const idx_t row_ = i_ / half_;                                          \
-        const idx_t p_ = i_ % half_;                                            \
+    _Pragma("omp parallel for collapse(2) num_threads(team()) schedule(static)") \
+    for (idx_t row_ = 0; row_ < rows_; row_++)                                  \
+    for (idx_t p_ = 0; p_ < half_; p_++) {                                      \
         const idx_t pos0 = row_ * dim_ + (((p_ & ~lowmask_) << 1) | (p_ & lowmask_)); \
         const idx_t pos1 = pos0 + ((idx_t)1 << (Q));
 #define PAIR_LOOP_END }
@@ -282,15 +292,14 @@ static void p_cx(rows_t* R, int ctrl, int tgt)
     const int lo = ctrl > tgt ? tgt : ctrl;
     const idx_t dim = (idx_t)1 << R->n;
     const idx_t quarter = dim >> 2;
-    const idx_t total = quarter * R->rows;
+    const idx_t rows = R->rows;
     const idx_t lo_mask = ((idx_t)1 << lo) - 1;
     const idx_t mid_mask = ((idx_t)1 << hi) - 1;
     double* const dr = R->re;
     double* const di = R->im;
-#pragma omp parallel for num_threads(g_threads) schedule(static)
-    for (idx_t i = 0; i < total; i++) {
-        const idx_t row = i / quarter;
-        const idx_t k = i % quarter;
+#pragma omp parallel for collapse(2) num_threads(team()) schedule(static)
+    for (idx_t row = 0; row < rows; row++)
+    for (idx_t k = 0; k < quarter; k++) {
         /* insert zeros at bit lo, then at bit hi */
         idx_t b = ((k & ~lo_mask) << 1) | (k & lo_mask);
         b = ((b & ~mid_mask) << 1) | (b & mid_mask);
@@ -311,15 +320,14 @@ static void p_c2(rows_t* R, int q1, int q2, const double* e)
     const int lo = q1 > q2 ? q2 : q1;
     const idx_t dim = (idx_t)1 << R->n;
     const idx_t quarter = dim >> 2;
-    const idx_t total = quarter * R->rows;
+    const idx_t rows = R->rows;
     const idx_t lo_mask = ((idx_t)1 << lo) - 1;
     const idx_t mid_mask = ((idx_t)1 << hi) - 1;
     double* const dr = R->re;
     double* const di = R->im;
-#pragma omp parallel for num_threads(g_threads) schedule(static)
-    for (idx_t i = 0; i < total; i++) {
-        const idx_t row = i / quarter;
-        const idx_t k = i % quarter;
+#pragma omp parallel for collapse(2) num_threads(team()) schedule(static)
+    for (idx_t row = 0; row < rows; row++)
+    for (idx_t k = 0; k < quarter; k++) {
         idx_t b = ((k & ~lo_mask) << 1) | (k & lo_mask);
         b = ((b & ~mid_mask) << 1) | (b & mid_mask);
         const idx_t base = row * dim + b;
@@ -536,6 +544,26 @@ int orc_forward(int n, idx_t rows, double* re, double* im, int ng, const int32_t
     return 0;
 }
 
+/* The same forward pass in another loop order: each thread applies every
+ * gate to one row before taking the next.  Rows do not interact in the
+ * forward pass and every entry sees the same operations in the same order,
+ * so the result is bit-identical to orc_forward; a row (2^n entries) stays in
+ * cache across the gates instead of every gate streaming all rows. */
+int orc_forward_rowwise(int n, idx_t rows, double* re, double* im, int ng, const int32_t* ops,
+                        const uint32_t* qb, const double* prm, const double* coef)
+{
+    const int bad = check_gates(n, ng, ops, qb);
+    if (bad) return bad;
+    const idx_t dim = (idx_t)1 << n;
+#pragma omp parallel for num_threads(g_threads) schedule(dynamic, 1)
+    for (idx_t r = 0; r < rows; r++) {
+        rows_t R = {re + r * dim, im + r * dim, n, 1};
+        for (int g = 0; g < ng; g++)
+            apply_gate(&R, ops[g], qb + 5 * g, prm + 3 * g, coef ? coef + 32 * g : NULL);
+    }
+    return 0;
+}
+
 /* In-place conjugate transpose of a dim x dim split-plane matrix. */
 void orc_conj_transpose(int n, double* re, double* im)
 {
@@ -562,6 +590,17 @@ int orc_sim(int n, double* re, double* im, int ng, const int32_t* ops, const uin
     if (rc) return rc;
     orc_conj_transpose(n, re, im);
     return orc_forward(n, dim, re, im, ng, ops, qb, prm, coef);
+}
+
+/* orc_sim with the row-wise forward pass (bit-identical, faster at large n). */
+int orc_sim_rowwise(int n, double* re, double* im, int ng, const int32_t* ops, const uint32_t* qb,
+                    const double* prm, const double* coef)
+{
+    const idx_t dim = (idx_t)1 << n;
+    int rc = orc_forward_rowwise(n, dim, re, im, ng, ops, qb, prm, coef);
+    if (rc) return rc;
+    orc_conj_transpose(n, re, im);
+    return orc_forward_rowwise(n, dim, re, im, ng, ops, qb, prm, coef);
 }
 
 /* |0..0><0..0| */

@@ -58,6 +58,8 @@ class Oracle:
         lib.orc_forward.restype = C.c_int
         lib.orc_sim.argtypes = [C.c_int, _dp(), _dp(), C.c_int, _ip(), _up(), _dp(), _dp()]
         lib.orc_sim.restype = C.c_int
+        lib.orc_sim_rowwise.argtypes = lib.orc_sim.argtypes
+        lib.orc_sim_rowwise.restype = C.c_int
         lib.orc_conj_transpose.argtypes = [C.c_int, _dp(), _dp()]
         lib.orc_init.argtypes = [C.c_int, _dp(), _dp()]
         lib.orc_measure.argtypes = [C.c_int, _dp(), C.c_uint32, C.c_uint32, _up()]
@@ -79,13 +81,16 @@ class Oracle:
         re[0] = 1.0
         return re, im
 
-    def sim(self, n, table, state=None):
+    def sim(self, n, table, state=None, rowwise=False):
         """One sim() run on ``state`` (default |0><0|); returns (re, im)
-        flattened dim*dim arrays in the reference's result orientation."""
+        flattened dim*dim arrays in the reference's result orientation.
+        rowwise: the forward passes row by row (bit-identical, cache-friendly
+        at large n)."""
         ops, qb, prm, coef = table
         re, im = (self.ground(n) if state is None else
                   (np.array(state[0], dtype=np.float64), np.array(state[1], dtype=np.float64)))
-        rc = self.lib.orc_sim(n, re, im, len(ops), ops, qb.ravel(), prm.ravel(), coef.ravel())
+        fn = self.lib.orc_sim_rowwise if rowwise else self.lib.orc_sim
+        rc = fn(n, re, im, len(ops), ops, qb.ravel(), prm.ravel(), coef.ravel())
         if rc:
             raise ValueError(f"oracle rejected gate {-rc - 1}")
         return re, im

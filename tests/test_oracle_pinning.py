@@ -90,6 +90,61 @@ def test_golden_mixed_n8_bitwise_and_shots(oracle):
     assert np.array_equal(re, d["s1_re"])
 
 
+def test_rowwise_forward_against_golden(oracle):
+    """orc_sim_rowwise (the forward passes row by row) reproduces the golden
+    vectors and the adder's shots bit for bit."""
+    import hashlib
+    d = load("circuits_n6.npz")
+    for s in range(4):
+        re, im = oracle.sim(6, table_from(d, f"c{s}_"), rowwise=True)
+        assert np.array_equal(re, d[f"c{s}_re"]) and np.array_equal(im, d[f"c{s}_im"]), s
+    d = load("mixed_n8.npz")
+    for s in range(1, 9):
+        re, im = oracle.sim(8, table_from(d, f"s{s}_"), rowwise=True)
+        assert hashlib.sha256(re.astype("<f8").tobytes()).hexdigest() == str(d[f"s{s}_sha_re"])
+        assert hashlib.sha256(im.astype("<f8").tobytes()).hexdigest() == str(d[f"s{s}_sha_im"])
+    d = load("adder_n10.npz")
+    re, _ = oracle.sim(10, table_from(d), rowwise=True)
+    assert hashlib.sha256(re.astype("<f8").tobytes()).hexdigest() == str(d["sha_re"])
+
+
+def test_rowwise_forward_equals_gatewise(oracle):
+    """Loop order only: every op (with C1/C2 matrices), continued runs, and
+    several thread counts give identical bits."""
+    for n, seed, threads_set in ((5, 1, (1, 3, 8)), (9, 2, (8,)), (10, 3, (5,))):
+        gate_set = [g for g in W.OP_NAMES if W.ARITY[g] <= n]
+        t = W.random_mixed(n, 120, 700 + seed, gate_set=gate_set).arrays()
+        t2 = W.brickwall_su4(n=n, layers=2, tail=30, seed=seed).arrays()
+        for threads in threads_set:
+            oracle.set_threads(threads)
+            a = oracle.sim(n, t2, oracle.sim(n, t))
+            b = oracle.sim(n, t2, oracle.sim(n, t, rowwise=True), rowwise=True)
+            assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1]), (n, threads)
+    oracle.set_threads(1)
+
+
+def test_measure_diag_equals_measure(oracle):
+    """orc_measure_diag on rho's diagonal draws exactly what orc_measure draws
+    on rho (and thereby the reference's golden shots)."""
+    d = load("mixed_n8.npz")
+    for s in range(1, 9):
+        re, _ = oracle.sim(8, table_from(d, f"s{s}_"))
+        seed = int(d[f"s{s}_shot_seed"])
+        diag = re.reshape(256, 256).diagonal().copy()
+        a, ta = oracle.measure(8, re, 64, seed)
+        b, tb = oracle.measure_diag(8, diag, 64, seed)
+        assert np.array_equal(a, b) and np.array_equal(b, d[f"s{s}_shots"]) and ta == tb, s
+    d = load("adder_n10.npz")
+    re, _ = oracle.sim(10, table_from(d))
+    shots, _ = oracle.measure_diag(10, re.reshape(1024, 1024).diagonal().copy(), 10,
+                                   int(d["shot_seed"]))
+    assert list(shots) == list(d["shots"])
+    # a diagonal with its mass moved must change the draw (the scan reads diag)
+    diag = np.zeros(256)
+    diag[17] = 1.0
+    assert set(oracle.measure_diag(8, diag, 32, 5)[0]) == {17}
+
+
 def test_golden_tables_regenerate():
     """The workload generator still produces the circuits the fixtures hold."""
     d = load("mixed_n8.npz")
