@@ -238,6 +238,17 @@ int dmx_get_entries(dmx_sim* sim, uint64_t count, const uint64_t* idx, double* r
     return guarded([&] { return sim->eng.get_entries(count, idx, re, im); });
 }
 
+int dmx_get_entries_noalloc(dmx_sim* sim, uint64_t count, const uint64_t* idx, double* re,
+                            double* im)
+{
+    if (!sim) return null_handle();
+    if (count && !idx) {
+        dmx::set_error("null index array");
+        return DMX_EINVAL;
+    }
+    return guarded([&] { return sim->eng.get_entries_noalloc(count, idx, re, im); });
+}
+
 int dmx_get_diag(dmx_sim* sim, double* diag)
 {
     if (!sim) return null_handle();

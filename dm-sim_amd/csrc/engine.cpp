@@ -11,6 +11,7 @@
 
 #include "dmsim_amd.h"
 #include "fuse.hpp"
+#include "noalloc.hpp"
 #include "fused.hpp"
 #include "mlp.hpp"
 #include "sweep.hpp"
@@ -71,6 +72,8 @@ int Engine::init(uint32_t n_qubits, uint32_t n_gpus)
     HIPTRY(hipMalloc(&d_im_, num_ * sizeof(double)));
     HIPTRY(hipMalloc(&d_diag_, dim_ * sizeof(double)));
     HIPTRY(plain_persistent_grid(n_, &persistent_blocks_));
+    void* scratch = nullptr;  // the allocation-free gather's staging on this device
+    HIPTRY(noalloc_scratch(&scratch));
     return reset_dm();
 }
 
@@ -511,6 +514,22 @@ int Engine::get_entries(uint64_t count, const uint64_t* idx, double* re, double*
     if (e == hipSuccess) e = hipStreamSynchronize(stream_);
     (void)hipFree(buf);
     HIPTRY(e);
+    return DMX_OK;
+}
+
+int Engine::get_entries_noalloc(uint64_t count, const uint64_t* idx, double* re, double* im)
+{
+    if (count > kNoAllocMaxEntries) {
+        set_error("at most " + std::to_string(kNoAllocMaxEntries) +
+                  " entries per allocation-free gather");
+        return DMX_EINVAL;
+    }
+    for (uint64_t t = 0; t < count; t++)
+        if (idx[t] >= num_) {
+            set_error("entry index out of range (must be < 4^n)");
+            return DMX_EINVAL;
+        }
+    HIPTRY(gather_noalloc(d_re_, d_im_, n_, idx, count, transposed_, re, im, stream_));
     return DMX_OK;
 }
 

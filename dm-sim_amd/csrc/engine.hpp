@@ -29,6 +29,9 @@ public:
     int measure(uint32_t reps, uint32_t seed, uint32_t* out, double* total);
     int get_dm(double* re, double* im);
     int get_entries(uint64_t count, const uint64_t* idx, double* re, double* im);
+    // The same with its device staging in a fixed per-device array (no
+    // allocation): count <= kNoAllocMaxEntries; overlapping calls on one GPU race.
+    int get_entries_noalloc(uint64_t count, const uint64_t* idx, double* re, double* im);
     int get_diag(double* diag);
     int expect_z(double* out);
     int set_method(int m);

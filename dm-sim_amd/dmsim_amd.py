@@ -17,6 +17,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libdmsim_amd.so")
 
 DMX_OK, DMX_EINVAL, DMX_ESTATE, DMX_EHIP, DMX_ENOMEM = 0, -1, -2, -3, -4
+NOALLOC_MAX_ENTRIES = 65536  # dmx_get_entries_noalloc limit (include/dmsim_amd.h)
 METHOD_PLAIN, METHOD_PLAIN_LAUNCH, METHOD_FUSED = 0, 1, 2
 COST_FUSED, COST_DENSE, COST_DIAG, COST_CX, COST_QUAD, COST_FUSED5 = 0, 1, 2, 3, 4, 5
 COST_FEATURES, COST_PARAMS = 13, 5121
@@ -26,7 +27,7 @@ EXPORTS = [
     "dmx_create", "dmx_destroy", "dmx_append", "dmx_append_matrix", "dmx_append_kraus",
     "dmx_upload", "dmx_run",
     "dmx_clear_circuit", "dmx_reset", "dmx_reset_dm", "dmx_measure", "dmx_get_dm",
-    "dmx_get_entries",
+    "dmx_get_entries", "dmx_get_entries_noalloc",
     "dmx_get_diag", "dmx_expect_z", "dmx_info", "dmx_dump", "dmx_set_method",
     "dmx_set_fusion_max_k", "dmx_set_fusion_permute", "dmx_last_run_stats", "dmx_last_run_fallbacks", "dmx_set_report",
     "dmx_set_profiling", "dmx_profile_get",
@@ -72,6 +73,7 @@ def load():
     lib.dmx_get_diag.argtypes = [vp, P(C.c_double)]
     lib.dmx_get_entries.argtypes = [vp, C.c_uint64, P(C.c_uint64), P(C.c_double),
                                     P(C.c_double)]
+    lib.dmx_get_entries_noalloc.argtypes = lib.dmx_get_entries.argtypes
     lib.dmx_expect_z.argtypes = [vp, P(C.c_double)]
     lib.dmx_info.argtypes = [vp, P(C.c_uint32), P(C.c_uint64), P(C.c_uint32)]
     lib.dmx_dump.argtypes = [vp, C.c_char_p, C.c_size_t, P(C.c_size_t)]
@@ -236,6 +238,16 @@ class Sim:
         im = np.empty(idx.size)
         _check(load().dmx_get_entries(self._h, idx.size, _ptr(idx, C.c_uint64),
                                       _ptr(re, C.c_double), _ptr(im, C.c_double)))
+        return re + 1j * im
+
+    def get_entries_noalloc(self, idx):
+        """get_entries through the allocation-free entry point (at most
+        NOALLOC_MAX_ENTRIES indices per call)."""
+        idx = np.ascontiguousarray(idx, dtype=np.uint64)
+        re = np.empty(idx.size)
+        im = np.empty(idx.size)
+        _check(load().dmx_get_entries_noalloc(self._h, idx.size, _ptr(idx, C.c_uint64),
+                                              _ptr(re, C.c_double), _ptr(im, C.c_double)))
         return re + 1j * im
 
     def get_diag(self):

@@ -121,6 +121,16 @@ int dmx_get_dm(dmx_sim* sim, double* re, double* im);
  * re or im may be NULL. */
 int dmx_get_entries(dmx_sim* sim, uint64_t count, const uint64_t* idx, double* re, double* im);
 
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (tight serving loops, harnesses that forbid hipMalloc): the same
+ * gather as dmx_get_entries, whose per-call device staging (indices and both
+ * outputs) is here a fixed array that each GPU the library runs on holds
+ * until the process ends; it calls no hipMalloc/hipFree.  At most 65536
+ * entries per call (1.5 MiB of staging), DMX_EINVAL beyond that.  Calls on
+ * one GPU share the array: calls that overlap in time on one GPU race. */
+int dmx_get_entries_noalloc(dmx_sim* sim, uint64_t count, const uint64_t* idx, double* re,
+                            double* im);
+
 /* Re rho_ii, i < 2^n (the diagonal measure() and print_res_sv() read,
  * src/dmsim_amdgpu_omp.hpp:541-543, :565-575). */
 int dmx_get_diag(dmx_sim* sim, double* diag);
