@@ -315,6 +315,16 @@ int dmx_set_fusion_permute(dmx_sim* sim, uint32_t on)
     return guarded([&] { return sim->eng.set_fusion_permute(on != 0); });
 }
 
+int dmx_set_fusion_hermitian(dmx_sim* sim, uint32_t mode)
+{
+    if (!sim) return null_handle();
+    if (mode > 2) {
+        dmx::set_error("fusion Hermitian mode must be 0, 1 or 2");
+        return DMX_EINVAL;
+    }
+    return guarded([&] { return sim->eng.set_fusion_hermitian((int)mode); });
+}
+
 int dmx_last_run_stats(const dmx_sim* sim, double* ms, uint32_t* launches, uint32_t* sweeps)
 {
     if (!sim) return null_handle();
@@ -568,6 +578,17 @@ int dmx_plan_info(const dmx_sim* sim, uint32_t* fused_blocks, uint32_t* plain_bl
     if (greedy_us) *greedy_us = pi.greedy_us;
     if (tiles)
         for (int m = 4; m <= 8; m++) tiles[m - 4] = pi.tile_hist[m];
+    return DMX_OK;
+}
+
+int dmx_plan_sweeps(const dmx_sim* sim, uint32_t* perm_blocks, uint32_t* herm_blocks,
+                    uint32_t* relabels)
+{
+    if (!sim) return null_handle();
+    const dmx::PlanInfo& pi = sim->eng.plan_info();
+    if (perm_blocks) *perm_blocks = pi.perm_blocks;
+    if (herm_blocks) *herm_blocks = pi.herm_blocks;
+    if (relabels) *relabels = pi.relabels;
     return DMX_OK;
 }
 

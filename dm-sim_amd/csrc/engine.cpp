@@ -157,10 +157,11 @@ int Engine::upload()
     }
     if (method_ == DMX_METHOD_FUSED) {
         // no plan (SRN in the circuit, or n < 4): the run uses the plain sweeps
+        const bool herm = herm_wanted();
         int rc = build_fused_plan(circuit_, n_, fusion_max_k_, model_, tile_bits_,
-                                  fusion_permute_, &plan_, &plan_info_);
+                                  fusion_permute_, &plan_, &plan_info_, herm);
         if (rc) return rc;
-        if (plan_ && plan_info_.perm_blocks > 0 && n_ < kPermuteSureQubits) {
+        if (plan_ && plan_info_.perm_blocks > 0 && n_ < kPermuteSureQubits && fusion_herm_ != 2) {
             // a short sweep is latency-bound and a permuting one pays its index
             // map's set-up: keep the permuting plan only if it still wins
             FusedPlan* inplace = nullptr;
@@ -209,6 +210,12 @@ int Engine::upload()
     HIPTRY(hipStreamSynchronize(stream_));
     uploaded_ = true;
     return DMX_OK;
+}
+
+bool Engine::herm_wanted() const
+{
+    if (!fusion_permute_ || fusion_herm_ == 0) return false;
+    return n_ >= (fusion_herm_ == 2 ? kHermMinQubits : kHermAutoQubits);
 }
 
 bool Engine::permuting_pays(const PlanInfo& perm, const PlanInfo& inplace) const

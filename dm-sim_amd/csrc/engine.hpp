@@ -43,6 +43,14 @@ public:
         fusion_permute_ = on;
         return 0;
     }
+    // Hermitian sweeps for four-qubit blocks (with permuting blocks on):
+    // 0 off, 1 from kHermAutoQubits qubits (default), 2 from 8 qubits.  Takes
+    // effect at the next upload().
+    int set_fusion_hermitian(int mode)
+    {
+        fusion_herm_ = mode;
+        return 0;
+    }
     // Cost model for the fused planner (not owned; nullptr = greedy plan).
     // Takes effect at the next upload().
     void set_cost_model(CostModel* model) { model_ = model; }
@@ -100,6 +108,8 @@ private:
     int method_ = 0;
     int fusion_max_k_ = 4;
     bool fusion_permute_ = true;
+    int fusion_herm_ = 1;
+    bool herm_wanted() const;
     bool permuting_pays(const PlanInfo& perm, const PlanInfo& inplace) const;
     int persistent_blocks_ = 0;
     int max_blocks_ = 8192;  // grid cap for one-primitive launches

@@ -42,7 +42,18 @@ using cd = std::complex<double>;
 // unfused, applied to the fast index and then to the slow index (valid for
 // complex-linear gates, whose row and column actions commute); or a channel
 // (one superoperator sweep, in the record for rho's current orientation).
-enum StepType { ST_FUSED4 = 0, ST_PLAIN = 1, ST_FUSED5 = 2, ST_CHANNEL = 3 };
+// Hermitian plans add two out-of-place steps: a four-qubit block through
+// k_fused4_herm (its record rewritten for the bit order rho has then), and a
+// relabelling of rho's index bits (k_relabel) back to the circuit's order
+// before any other step and at the end of the run.
+enum StepType {
+    ST_FUSED4 = 0,
+    ST_PLAIN = 1,
+    ST_FUSED5 = 2,
+    ST_CHANNEL = 3,
+    ST_HERM = 4,
+    ST_RELABEL = 5
+};
 
 struct Step {
     int type;
@@ -89,8 +100,28 @@ struct Plan : FusedPlan {
     bool needs_alt() const override
     {
         for (const Step& st : steps)
-            if (st.type == ST_FUSED4 && blocks[st.index].perm_on) return true;
+            if ((st.type == ST_FUSED4 && blocks[st.index].perm_on) || st.type == ST_HERM ||
+                st.type == ST_RELABEL)
+                return true;
         return false;
+    }
+
+    void describe(std::vector<PlanStepView>& out) const override
+    {
+        out.clear();
+        for (const Step& st : steps) {
+            PlanStepView v;
+            v.kind = st.type == ST_HERM      ? PlanStepView::HERM
+                     : st.type == ST_RELABEL ? PlanStepView::RELABEL
+                     : st.type == ST_FUSED4  ? PlanStepView::FUSED4
+                     : st.type == ST_FUSED5  ? PlanStepView::FUSED5
+                                             : PlanStepView::OTHER;
+            v.block = (st.type == ST_HERM || st.type == ST_RELABEL || st.type == ST_FUSED4)
+                          ? &blocks[st.index]
+                          : nullptr;
+            v.block5 = st.type == ST_FUSED5 ? &blocks5[st.index] : nullptr;
+            out.push_back(v);
+        }
     }
 
     int upload(hipStream_t stream) override
@@ -108,7 +139,22 @@ struct Plan : FusedPlan {
         uint32_t la = 0;
         for (const Step& st : steps) {
             hipError_t e = hipSuccess;
-            if (st.type == ST_FUSED4 && blocks[st.index].perm_on) {
+            if (st.type == ST_HERM || st.type == ST_RELABEL) {
+                if (!re2 || !im2) {
+                    set_error("internal: an out-of-place step needs the second pair of planes");
+                    return DMX_EINVAL;
+                }
+                const bool h = st.type == ST_HERM;
+                prof.begin(h ? KC_FUSED4H : KC_RELABEL, h ? herm_bytes(nq) : sweep_bytes);
+                e = h ? launch_fused4_herm(re, im, re2, im2, d_blocks + st.index, nq, max_blocks,
+                                           stream)
+                      : launch_relabel(re, im, re2, im2, d_blocks + st.index, nq, max_blocks,
+                                       stream);
+                prof.end();
+                std::swap(re, re2);
+                std::swap(im, im2);
+                la++;
+            } else if (st.type == ST_FUSED4 && blocks[st.index].perm_on) {
                 if (!re2 || !im2) {
                     set_error("internal: a permuting block needs the second pair of planes");
                     return DMX_EINVAL;
@@ -312,8 +358,8 @@ void commute_classes(const Gate& g, uint32_t& zm, uint32_t& xm)
 // first ungrouped gate of each qubit.
 class FrontierGrouper {
 public:
-    FrontierGrouper(const std::vector<Gate>& circuit, int n, int kmax, bool allow_perm)
-        : circuit_(circuit), n_(n), kmax_(kmax), lists_(n), head_(n, 0),
+    FrontierGrouper(const std::vector<Gate>& circuit, int n, int kmax, bool allow_perm, bool herm)
+        : circuit_(circuit), n_(n), kmax_(kmax), herm_(herm), lists_(n), head_(n, 0),
           mask_(circuit.size(), 0), zm_(circuit.size(), 0), xm_(circuit.size(), 0),
           pos0_(circuit.size(), 0), done_(circuit.size(), 0), mark_(circuit.size(), 0),
           listed_(circuit.size(), 0), base_(n), work_(n)
@@ -489,6 +535,24 @@ private:
         }
     }
 
+    // Hermitian sweeps: the index map must send the span of the block's bits
+    // (img: where the trailing gates so far have moved them) onto a set of
+    // index bits.  X and SWAP always do (SWAP moves img); CX does unless its
+    // control is in img and its target is not.
+    bool keeps_bits(int gi, uint32_t& img) const
+    {
+        const Gate& g = circuit_[gi];
+        const uint32_t a = 1u << g.qb[0], b = 1u << g.qb[1];
+        switch (g.op) {
+        case OP_X: return true;
+        case OP_CX: return !((img & a) && !(img & b));
+        case OP_SWAP:
+            if (((img & a) != 0) != ((img & b) != 0)) img ^= a | b;
+            return true;
+        default: return false;
+        }
+    }
+
     // Appends to a four-qubit block the permutation gates that are ready
     // after it (repeatedly), as its trailing index map; touched gains their
     // qubits.
@@ -497,6 +561,7 @@ private:
         uint32_t qm = pb.mask;
         for (int q = 0; q < n_ && __builtin_popcount(qm) < 4; q++) qm |= 1u << q;
         if (__builtin_popcount(qm) != 4) return;
+        uint32_t img = qm;
         uint32_t cm = qm;  // super-tile column bits at the default width
         for (int q = 0; q < n_ && __builtin_popcount(cm) < default_tile_bits(n_); q++)
             cm |= 1u << q;
@@ -512,7 +577,8 @@ private:
             ready_on(all, ready);
             for (int h : ready) {
                 const int op = circuit_[h].op;
-                if (op >= OP_COUNT || !perm_op_[op] || !keeps_runs(h, r)) continue;
+                if (op >= OP_COUNT || !perm_op_[op] || !(herm_ ? keeps_bits(h, img) : keeps_runs(h, r)))
+                    continue;
                 mark_[h] = stamp_;
                 pb.gates.push_back(h);
                 touched |= mask_[h];
@@ -682,6 +748,7 @@ private:
     static constexpr size_t kClassCacheMax = 4096;
     const std::vector<Gate>& circuit_;
     const int n_, kmax_;
+    const bool herm_;  // trailing gates must keep the block's bits a set of index bits
     std::vector<std::vector<int>> lists_;  // per qubit: its gates in program order
     std::vector<int> head_;                // per qubit: first gate not yet grouped
     std::vector<uint32_t> mask_, zm_, xm_; // per circuit index: qubits, commutation classes
@@ -697,10 +764,10 @@ private:
     bool perm_op_[OP_COUNT] = {};  // ops that may join a block's index map
 };
 
-void group_frontier(const std::vector<Gate>& circuit, int n, int kmax, bool allow_perm,
+void group_frontier(const std::vector<Gate>& circuit, int n, int kmax, bool allow_perm, bool herm,
                     std::vector<PB>& pbs, Seq& seq)
 {
-    FrontierGrouper fg(circuit, n, kmax, allow_perm);
+    FrontierGrouper fg(circuit, n, kmax, allow_perm, herm);
     std::vector<int> seg;
     for (int gi = 0; gi < (int)circuit.size(); gi++) {
         const Gate& g = circuit[gi];
@@ -728,7 +795,7 @@ double grouping_cost(const std::vector<PB>& pbs)
 }  // namespace
 
 int plan_blocks(const std::vector<Gate>& circuit, int n, int max_k, BlockPlan& out, bool& fusable,
-                bool allow_perm)
+                bool allow_perm, bool herm)
 {
     out = BlockPlan{};
     fusable = false;
@@ -741,7 +808,7 @@ int plan_blocks(const std::vector<Gate>& circuit, int n, int max_k, BlockPlan& o
     std::vector<PB> pbs, pbs2;
     Seq seq, seq2;
     group_least_growth(circuit, kmax, pbs, seq);
-    group_frontier(circuit, n, kmax, allow_perm, pbs2, seq2);
+    group_frontier(circuit, n, kmax, allow_perm, herm, pbs2, seq2);
     if (grouping_cost(pbs2) < grouping_cost(pbs)) {
         pbs.swap(pbs2);
         seq.swap(seq2);
@@ -793,7 +860,7 @@ int plan_blocks(const std::vector<Gate>& circuit, int n, int max_k, BlockPlan& o
                 fb.perm_on = 1;
                 fb.perm_b = pi(0);
                 for (int q = 0; q < n; q++) fb.perm_L[q] = pi(1u << q) ^ fb.perm_b;
-                if (fb.perm_L[0] != 1u) {  // the planner keeps bit 0; the kernel relies on it
+                if (!herm && fb.perm_L[0] != 1u) {  // k_fused4_perm relies on bit 0 staying
                     set_error("internal: a block's permutation moves index bit 0");
                     return DMX_EINVAL;
                 }
@@ -971,16 +1038,127 @@ int choose_fused5(const std::vector<Gate>& circuit, int n, BlockPlan& bp, CostMo
     return DMX_OK;
 }
 
+// Rewrites a plan's four-qubit blocks as Hermitian sweeps.  sigma[q] is the
+// index bit that holds logical qubit q (the identity at the start).  A block
+// on logical qubits Q with trailing map pi (logical, x -> Lp x ^ bp) reads
+// rho in the order sigma and writes it in a new order sigma': the qubits
+// spanning pi(span Q) at bits 0..3 (ascending), then the images of the
+// super-tile's four low bits, then the rest by their current bit.  Its
+// record gets the block's bits and matrix in the current order and the map
+// phi = sigma' o pi o sigma^-1 on index bits.  Before any other step and at
+// the end, a relabelling step restores the circuit's order.
+void to_hermitian(Plan& plan, int n, PlanInfo& info)
+{
+    std::vector<Step> steps;
+    std::vector<int> sigma(n), sinv(n);
+    for (int q = 0; q < n; q++) sigma[q] = sinv[q] = q;
+    auto at_home = [&] {
+        for (int q = 0; q < n; q++)
+            if (sigma[q] != q) return false;
+        return true;
+    };
+    auto move_bits = [&](const std::vector<int>& to, uint32_t v) {
+        uint32_t o = 0;
+        for (int q = 0; q < n; q++)
+            if ((v >> q) & 1u) o |= 1u << to[q];
+        return o;
+    };
+    auto go_home = [&] {
+        if (at_home()) return;
+        FusedBlock fb;
+        std::memset(&fb, 0, sizeof(fb));
+        fb.perm_on = 1;
+        for (int p = 0; p < n; p++) fb.perm_L[p] = 1u << sinv[p];  // bit p holds qubit sinv[p]
+        steps.push_back({ST_RELABEL, (int)plan.blocks.size(), 0});
+        plan.blocks.push_back(fb);
+        info.relabels++;
+        for (int q = 0; q < n; q++) sigma[q] = sinv[q] = q;
+    };
+    for (const Step& st : plan.steps) {
+        if (st.type != ST_FUSED4) {
+            go_home();
+            steps.push_back(st);
+            continue;
+        }
+        const FusedBlock b = plan.blocks[st.index];
+        uint32_t Lp[16] = {};
+        for (int q = 0; q < n; q++) Lp[q] = b.perm_on ? b.perm_L[q] : 1u << q;
+        const uint32_t bp = b.perm_on ? b.perm_b : 0u;
+        uint32_t img = 0;
+        for (uint32_t m = b.qmask; m; m &= m - 1) img |= Lp[__builtin_ctz(m)];
+        if (__builtin_popcount(img) != 4) {  // pi(span Q) is no set of bits: in-order kernel
+            go_home();
+            steps.push_back(st);
+            continue;
+        }
+        const uint32_t qphys = move_bits(sigma, b.qmask);
+        uint32_t cm = qphys;
+        for (int q = 0; q < n && __builtin_popcount(cm) < 8; q++) cm |= 1u << q;
+        std::vector<int> order;
+        std::vector<char> placed(n, 0);
+        auto place = [&](int q) {
+            if (!placed[q]) {
+                placed[q] = 1;
+                order.push_back(q);
+            }
+        };
+        for (int q = 0; q < n; q++)
+            if ((img >> q) & 1u) place(q);
+        for (int p = 0; p < n; p++)
+            if (((cm & ~qphys) >> p) & 1u) {
+                const uint32_t t = Lp[sinv[p]];
+                if (__builtin_popcount(t) == 1) place(__builtin_ctz(t));
+            }
+        for (int p = 0; p < n; p++) place(sinv[p]);
+        std::vector<int> so(n), soinv(n);
+        for (int i = 0; i < n; i++) {
+            so[order[i]] = i;
+            soinv[i] = order[i];
+        }
+        FusedBlock h;
+        std::memset(&h, 0, sizeof(h));
+        h.qmask = qphys;
+        h.m = 8;
+        h.perm_on = 1;
+        for (int p = 0; p < n; p++) h.perm_L[p] = move_bits(so, Lp[sinv[p]]);
+        h.perm_b = move_bits(so, bp);
+        // local bit t of the block's matrix (t-th lowest logical qubit) is
+        // local bit tphys[t] of the kernel's (t-th lowest index bit of qphys)
+        int tphys[4], t = 0;
+        for (uint32_t m = b.qmask; m; m &= m - 1)
+            tphys[t++] = __builtin_popcount(qphys & ((1u << sigma[__builtin_ctz(m)]) - 1u));
+        auto remap = [&](int a) {
+            int o = 0;
+            for (int u = 0; u < 4; u++)
+                if ((a >> u) & 1) o |= 1 << tphys[u];
+            return o;
+        };
+        for (int a = 0; a < 16; a++)
+            for (int c = 0; c < 16; c++) {
+                h.ur[remap(a) * 16 + remap(c)] = b.ur[a * 16 + c];
+                h.ui[remap(a) * 16 + remap(c)] = b.ui[a * 16 + c];
+            }
+        steps.push_back({ST_HERM, (int)plan.blocks.size(), 0});
+        plan.blocks.push_back(h);
+        info.herm_blocks++;
+        sigma = so;
+        sinv = soinv;
+    }
+    go_home();
+    plan.steps.swap(steps);
+}
+
 }  // namespace
 
 int build_fused_plan(const std::vector<Gate>& circuit, int n, int max_k, CostModel* model,
-                     int tile_bits, bool allow_perm, FusedPlan** out, PlanInfo* info)
+                     int tile_bits, bool allow_perm, FusedPlan** out, PlanInfo* info, bool herm)
 {
     *out = nullptr;
     PlanInfo pi;
     BlockPlan bp;
     bool fusable = false;
-    int rc = plan_blocks(circuit, n, max_k, bp, fusable, allow_perm);
+    herm = herm && allow_perm && n >= kHermMinQubits && n <= 16;
+    int rc = plan_blocks(circuit, n, max_k, bp, fusable, allow_perm, herm);
     if (rc || !fusable) return rc;
     std::vector<std::vector<PrimOp>> plain(bp.blocks.size());
     std::vector<SubPlan> sub(bp.blocks5.size());
@@ -1036,6 +1214,7 @@ int build_fused_plan(const std::vector<Gate>& circuit, int n, int max_k, CostMod
             plan->steps.push_back({ST_PLAIN, first, (int)plan->prims.size() - first});
         }
     }
+    if (herm) to_hermitian(*plan, n, pi);
     if (model && tile_bits <= 0 && !plan->blocks5.empty()) {
         // Whole-plan check: five-qubit blocks pay off only when they replace
         // enough four-qubit sweeps, which no single block's split can show.
@@ -1043,7 +1222,7 @@ int build_fused_plan(const std::vector<Gate>& circuit, int n, int max_k, CostMod
         // faster by the margin.
         FusedPlan* alt = nullptr;
         PlanInfo ai;
-        rc = build_fused_plan(circuit, n, 4, model, 0, allow_perm, &alt, &ai);
+        rc = build_fused_plan(circuit, n, 4, model, 0, allow_perm, &alt, &ai, herm);
         if (rc) {
             delete plan;
             return rc;

@@ -13,8 +13,25 @@
 
 namespace dmx {
 
+// One step of a fused plan as the executor runs it (host-side checks).
+struct PlanStepView {
+    enum Kind { FUSED4, FUSED5, HERM, RELABEL, OTHER } kind;
+    const FusedBlock* block;    // FUSED4, HERM, RELABEL: its record as launched
+    const FusedBlock5* block5;  // FUSED5
+};
+
+// Algorithmic bytes of one Hermitian block sweep (k_fused4_herm) on n >= 8
+// qubits: the super-tiles on and above the diagonal read (16 * H(H+1)/2 of
+// them, H = 2^(n-8), 64 KiB each), every entry written.
+inline double herm_bytes(int n)
+{
+    const double H = (double)(1ull << (n - 8));
+    return 16.0 * H * (H + 1.0) / 2.0 * 65536.0 + 16.0 * (double)(1ull << (2 * n));
+}
+
 struct FusedPlan {
     virtual ~FusedPlan() = default;
+    virtual void describe(std::vector<PlanStepView>& out) const = 0;
     virtual int upload(hipStream_t stream) = 0;
     // Whether execute needs a second pair of planes (blocks with a trailing
     // permutation run out of place).
@@ -50,8 +67,12 @@ struct BlockPlan {
 // allow_perm (n <= 16): a four-qubit block may also take the X, CX and SWAP
 // gates that become ready after it, run as a permutation of its output
 // (launch_fused4_perm, out of place).
+// herm: the plan runs Hermitian sweeps (k_fused4_herm), whose output map must
+// send the block's index bits onto bits 0..3: a trailing CX then joins only if
+// its control is not a block bit (after the block's earlier trailing gates)
+// while its target is not.
 int plan_blocks(const std::vector<Gate>& circuit, int n, int max_k, BlockPlan& out, bool& fusable,
-                bool allow_perm = false);
+                bool allow_perm = false, bool herm = false);
 
 // Relative extra cost of a sweep with a trailing permutation on n qubits:
 // about 2.5 us of index-map set-up against the sweep's own time (32 * 4^n bytes
@@ -85,6 +106,8 @@ struct PlanInfo {
                                  // four-qubit sub-blocks instead)
     uint32_t channels = 0;       // noise channels (one superoperator sweep each)
     uint32_t perm_blocks = 0;    // fused four-qubit blocks with a trailing permutation
+    uint32_t herm_blocks = 0;    // four-qubit blocks run as Hermitian sweeps
+    uint32_t relabels = 0;       // relabelling sweeps
     uint32_t tile_hist[9] = {};  // fused blocks per tile width m (4..8)
     double predicted_us = 0.0;
     double greedy_us = 0.0;
@@ -99,7 +122,17 @@ struct PlanInfo {
 // greedy plan).
 // tile_bits > 0 fixes every fused block's tile width where that width is
 // admissible for the block (tile_bits_ok), and takes precedence over the model.
+// herm (n = 8..16, with allow_perm): four-qubit blocks run as Hermitian
+// sweeps (k_fused4_herm) with rho's index bits relabelled so that each
+// block's output lands in aligned 16-column runs; rho returns to the
+// circuit's bit order (k_relabel) before any other step and at the end.
 int build_fused_plan(const std::vector<Gate>& circuit, int n, int max_k, CostModel* model,
-                     int tile_bits, bool allow_perm, FusedPlan** out, PlanInfo* info);
+                     int tile_bits, bool allow_perm, FusedPlan** out, PlanInfo* info,
+                     bool herm = false);
+
+// Smallest n for Hermitian plans, and the smallest where the engine uses
+// them by default.
+constexpr int kHermMinQubits = 8;
+constexpr int kHermAutoQubits = 12;
 
 }  // namespace dmx

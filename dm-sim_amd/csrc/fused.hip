@@ -25,6 +25,7 @@
 #include <cstdint>
 
 #include "fused.hpp"
+#include "herm.hpp"
 
 namespace dmx {
 
@@ -288,6 +289,188 @@ __global__ void __launch_bounds__(256, 2) k_fused4_batch(const BatchEntry* __res
         fused_sweep<M>(e.re, e.im, e.re, e.im, e.blk, n, blockIdx.x, gridDim.x, s_re, s_im);
 }
 
+// ---- Hermitian sweep ------------------------------------------------------
+//
+// rho stays Hermitian under every map the engine applies (U rho U^dagger,
+// Kraus sums), so the stored A = conj(rho) satisfies T(c, r) = T(r, c)^dagger
+// for the tiles of any block (r, c: the tiles' free row and column parts),
+// and T'(c, r) = T'(r, c)^dagger after the block as well.  This sweep reads
+// and transforms only the tiles with r <= c and writes each result twice:
+// T'(r, c) in the tile's place and its conjugate transpose in the mirror
+// tile's place -- half the reads and half the MFMA work of k_fused4.
+// Writes go to a second pair of planes through the block's output map phi,
+// which the host chooses so that every tile lands in 16 rows x one aligned
+// run of 16 columns both ways (index maps: herm.hpp).  phi also carries the
+// block's trailing X/CX/SWAP gates and the new bit order.
+__global__ void __launch_bounds__(256, 2) k_fused4_herm(const double* __restrict__ re,
+                                                         const double* __restrict__ im,
+                                                         double* __restrict__ re_out,
+                                                         double* __restrict__ im_out,
+                                                         const FusedBlock* __restrict__ blk,
+                                                         int n)
+{
+    __shared__ double s_re[16 * kHermPitch];
+    __shared__ double s_im[16 * kHermPitch];
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const int wave = tid >> 6;
+    const int j = lane & 15;
+    const int kk = lane >> 4;
+
+    HermGeom g;
+    g.init(*blk, n);
+
+    double ur[4], ui[4], upi[4], nui[4], umi[4];
+#pragma unroll
+    for (int c = 0; c < 4; c++) {
+        const int idx = j * 16 + 4 * c + kk;
+        ur[c] = blk->ur[idx];
+        ui[c] = blk->ui[idx];
+        upi[c] = ur[c] + ui[c];
+        nui[c] = -ui[c];
+        umi[c] = ur[c] - ui[c];
+    }
+    uint32_t lcq_a[4];
+#pragma unroll
+    for (int c = 0; c < 4; c++) lcq_a[c] = (uint32_t)deposit(4 * c + kk, g.pos_q);
+    const uint32_t lcq_d = (uint32_t)deposit(j, g.pos_q);
+
+    HermThread th;
+    th.init(g, tid);
+    const int lc = th.lc;
+    uint64_t w = 0;
+    for (uint64_t k = 0; herm_slot(g, blockIdx.x, gridDim.x, k, w); k++) {
+        const HermItem it = herm_item(g, w);
+        {
+            double2 vr[8], vi[8];
+            int a[8];
+#pragma unroll
+            for (int rep = 0; rep < 8; rep++) {
+                const uint64_t off = herm_load(g, th, it, rep, a[rep]);
+                vr[rep] = *reinterpret_cast<const double2*>(re + off);
+                vi[rep] = *reinterpret_cast<const double2*>(im + off);
+            }
+#pragma unroll
+            for (int rep = 0; rep < 8; rep++) {
+                s_re[a[rep] * kHermPitch + lc] = vr[rep].x;
+                s_re[a[rep] * kHermPitch + lc + 1] = vr[rep].y;
+                s_im[a[rep] * kHermPitch + lc] = vi[rep].x;
+                s_im[a[rep] * kHermPitch + lc + 1] = vi[rep].y;
+            }
+        }
+        __syncthreads();
+
+        for (int tb = wave; tb < 16; tb += 4) {
+            if (!herm_tile_on(it, tb)) continue;
+            const uint32_t base = (uint32_t)deposit((uint64_t)tb, g.pos_f);
+            double tr[4], ti[4], ts[4];
+#pragma unroll
+            for (int c = 0; c < 4; c++) {
+                tr[c] = s_re[j * kHermPitch + base + lcq_a[c]];
+                ti[c] = s_im[j * kHermPitch + base + lcq_a[c]];
+                ts[c] = tr[c] + ti[c];
+            }
+            d4 p1 = {0, 0, 0, 0}, p2 = {0, 0, 0, 0}, p3 = {0, 0, 0, 0};
+#pragma unroll
+            for (int c = 0; c < 4; c++) {
+                p1 = mfma(tr[c], ur[c], p1);
+                p2 = mfma(ti[c], ui[c], p2);
+                p3 = mfma(ts[c], upi[c], p3);
+            }
+            const d4 xr = p1 - p2;
+            const d4 xi = p3 - p1 - p2;
+            const d4 xs = xr + xi;
+            d4 q1 = {0, 0, 0, 0}, q2 = {0, 0, 0, 0}, q3 = {0, 0, 0, 0};
+#pragma unroll
+            for (int c = 0; c < 4; c++) {
+                q1 = mfma(ur[c], xr[c], q1);
+                q2 = mfma(nui[c], xi[c], q2);
+                q3 = mfma(umi[c], xs[c], q3);
+            }
+            const d4 yr = q1 - q2;
+            const d4 yi = q3 - q1 - q2;
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const int a = 4 * r + kk;
+                s_re[a * kHermPitch + base + lcq_d] = yr[r];
+                s_im[a * kHermPitch + base + lcq_d] = yi[r];
+            }
+        }
+        __syncthreads();
+
+#pragma unroll
+        for (int rep = 0; rep < 8; rep++) {
+            uint64_t off;
+            int a, c0, c1;
+            if (herm_direct(g, th, it, rep, off, a, c0, c1)) {
+                *reinterpret_cast<double2*>(re_out + off) =
+                    make_double2(s_re[a * kHermPitch + c0], s_re[a * kHermPitch + c1]);
+                *reinterpret_cast<double2*>(im_out + off) =
+                    make_double2(s_im[a * kHermPitch + c0], s_im[a * kHermPitch + c1]);
+            }
+        }
+#pragma unroll
+        for (int rep = 0; rep < 8; rep++) {
+            uint64_t off;
+            int a0, a1, col;
+            if (herm_mirror(g, th, it, rep, off, a0, a1, col)) {
+                *reinterpret_cast<double2*>(re_out + off) =
+                    make_double2(s_re[a0 * kHermPitch + col], s_re[a1 * kHermPitch + col]);
+                *reinterpret_cast<double2*>(im_out + off) =
+                    make_double2(-s_im[a0 * kHermPitch + col], -s_im[a1 * kHermPitch + col]);
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// rho's index bits relabelled, entry (x, y) -> (p(x), p(y)) for a bit
+// permutation p (perm_L[q] = 1 << p(q)), out of place.  A workgroup stages
+// 16 consecutive rows x 256 columns whose column bits hold bits 0..3 and the
+// bits p maps to 0..3, so reads and writes move runs of at least 16 doubles
+// (index maps: herm.hpp).
+__global__ void __launch_bounds__(256, 2) k_relabel(const double* __restrict__ re,
+                                                     const double* __restrict__ im,
+                                                     double* __restrict__ re_out,
+                                                     double* __restrict__ im_out,
+                                                     const FusedBlock* __restrict__ blk, int n)
+{
+    __shared__ double s_re[16 * kHermPitch];
+    __shared__ double s_im[16 * kHermPitch];
+    const int tid = threadIdx.x;
+    RelabelGeom g;
+    g.init(*blk, n);
+    RelabelThread th;
+    th.init(g, tid);
+    const int lc = th.lc;
+    const uint64_t n_items = g.items();
+    for (uint64_t w = blockIdx.x; w < n_items; w += gridDim.x) {
+        const RelabelItem it = relabel_item(g, w);
+#pragma unroll
+        for (int rep = 0; rep < 8; rep++) {
+            int a;
+            const uint64_t off = relabel_load(g, th, it, tid, rep, a);
+            const double2 vr = *reinterpret_cast<const double2*>(re + off);
+            const double2 vi = *reinterpret_cast<const double2*>(im + off);
+            s_re[a * kHermPitch + lc] = vr.x;
+            s_re[a * kHermPitch + lc + 1] = vr.y;
+            s_im[a * kHermPitch + lc] = vi.x;
+            s_im[a * kHermPitch + lc + 1] = vi.y;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int rep = 0; rep < 8; rep++) {
+            int a;
+            const uint64_t off = relabel_store(g, th, it, tid, rep, a);
+            *reinterpret_cast<double2*>(re_out + off) =
+                make_double2(s_re[a * kHermPitch + th.s0], s_re[a * kHermPitch + th.s1]);
+            *reinterpret_cast<double2*>(im_out + off) =
+                make_double2(s_im[a * kHermPitch + th.s0], s_im[a * kHermPitch + th.s1]);
+        }
+        __syncthreads();
+    }
+}
+
 namespace {
 
 uint64_t supertiles(int n, int m) { return (1ull << (n - 4)) * (1ull << (n - m)); }
@@ -325,6 +508,29 @@ hipError_t launch_fused4_perm(const double* re, const double* im, double* re_out
     case 7: k_fused4_perm<7><<<blocks, 256, 0, stream>>>(re, im, re_out, im_out, d_blk, n); break;
     default: k_fused4_perm<8><<<blocks, 256, 0, stream>>>(re, im, re_out, im_out, d_blk, n); break;
     }
+    return hipGetLastError();
+}
+
+hipError_t launch_fused4_herm(const double* re, const double* im, double* re_out,
+                              double* im_out, const FusedBlock* d_blk, int n, int max_blocks,
+                              hipStream_t stream)
+{
+    if (n < 8 || n > 16 || re == re_out || im == im_out) return hipErrorInvalidValue;
+    const uint64_t H = 1ull << (n - 8);
+    const uint64_t items = 16 * (H * (H + 1) / 2);
+    // a multiple of 128 workgroups (herm_slot): 16 per XCD group
+    uint64_t blocks = ((items < (uint64_t)max_blocks ? items : (uint64_t)max_blocks) + 127) / 128 * 128;
+    k_fused4_herm<<<(unsigned)blocks, 256, 0, stream>>>(re, im, re_out, im_out, d_blk, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_relabel(const double* re, const double* im, double* re_out, double* im_out,
+                          const FusedBlock* d_blk, int n, int max_blocks, hipStream_t stream)
+{
+    if (n < 8 || n > 16 || re == re_out || im == im_out) return hipErrorInvalidValue;
+    const uint64_t items = (1ull << (n - 4)) * (1ull << (n - 8));
+    const unsigned blocks = (unsigned)(items < (uint64_t)max_blocks ? items : (uint64_t)max_blocks);
+    k_relabel<<<blocks, 256, 0, stream>>>(re, im, re_out, im_out, d_blk, n);
     return hipGetLastError();
 }
 

@@ -77,6 +77,20 @@ hipError_t launch_fused4_perm(const double* re, const double* im, double* re_out
                               const FusedBlock* d_blk, int n, int m, int max_blocks,
                               hipStream_t stream);
 
+// Hermitian form of the block sweep for n = 8..16 (rho Hermitian): reads
+// the tiles on and above the diagonal of (re, im) and writes every entry of
+// (re_out, im_out), entry (i, j) of U rho U^dagger to (phi(i), phi(j)) with
+// phi(x) = L x + b (perm_L, perm_b).  L must map the span of the block's
+// qmask bits onto bits 0..3 (not checked here).  No overlap of input and output.
+hipError_t launch_fused4_herm(const double* re, const double* im, double* re_out,
+                              double* im_out, const FusedBlock* d_blk, int n, int max_blocks,
+                              hipStream_t stream);
+
+// rho's index bits relabelled by a permutation p (perm_L[q] = 1 << p(q)),
+// out of place, n = 8..16.
+hipError_t launch_relabel(const double* re, const double* im, double* re_out, double* im_out,
+                          const FusedBlock* d_blk, int n, int max_blocks, hipStream_t stream);
+
 // The same for `count` circuits of n qubits each (one launch of about
 // max_blocks workgroups in all).
 hipError_t launch_fused4_batch(const BatchEntry* d_entries, int count, int n, int m,

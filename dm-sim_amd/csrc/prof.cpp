@@ -5,7 +5,7 @@ namespace dmx {
 
 const char* const kKernelClassNames[KC_COUNT] = {
     "k_fused4", "k_sweep(dense pair)", "k_sweep(diagonal pair)", "k_sweep(cx)", "k_quad",
-    "k_plain_persistent", "k_fused5", "k_fused4_perm"};
+    "k_plain_persistent", "k_fused5", "k_fused4_perm", "k_fused4_herm", "k_relabel"};
 
 LaunchProfiler::~LaunchProfiler()
 {

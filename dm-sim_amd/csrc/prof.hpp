@@ -16,6 +16,8 @@ enum KernelClass : int {
     KC_PERSISTENT,    // k_plain_persistent, a segment of primitives
     KC_FUSED5,        // k_fused5: one dense 5-qubit block, both sides of rho
     KC_FUSED4P,       // k_fused4_perm: a 4-qubit block with a trailing permutation, out of place
+    KC_FUSED4H,       // k_fused4_herm: a 4-qubit block on the upper tiles of Hermitian rho, out of place
+    KC_RELABEL,       // k_relabel: rho's index bits relabelled, out of place
     KC_COUNT
 };
 

@@ -29,12 +29,12 @@ EXPORTS = [
     "dmx_clear_circuit", "dmx_reset", "dmx_reset_dm", "dmx_measure", "dmx_get_dm",
     "dmx_get_entries", "dmx_get_entries_noalloc",
     "dmx_get_diag", "dmx_expect_z", "dmx_info", "dmx_dump", "dmx_set_method",
-    "dmx_set_fusion_max_k", "dmx_set_fusion_permute", "dmx_last_run_stats", "dmx_last_run_fallbacks", "dmx_set_report",
+    "dmx_set_fusion_max_k", "dmx_set_fusion_permute", "dmx_set_fusion_hermitian", "dmx_last_run_stats", "dmx_last_run_fallbacks", "dmx_set_report",
     "dmx_set_profiling", "dmx_profile_get",
     "dmx_qasm_parse", "dmx_append_qasm",
     "dmx_cost_create", "dmx_cost_destroy", "dmx_cost_features", "dmx_cost_train",
     "dmx_cost_predict", "dmx_cost_gradient", "dmx_cost_get_params", "dmx_cost_set_params",
-    "dmx_set_cost_model", "dmx_set_tile_bits", "dmx_plan_info", "dmx_time_launch",
+    "dmx_set_cost_model", "dmx_set_tile_bits", "dmx_plan_info", "dmx_plan_sweeps", "dmx_time_launch",
     "dmx_batch_create", "dmx_batch_destroy", "dmx_batch_add_circuit", "dmx_batch_add_pauli",
     "dmx_batch_upload", "dmx_batch_set_graph", "dmx_batch_set_autotune", "dmx_batch_tile_bits",
     "dmx_batch_run", "dmx_batch_energies", "dmx_batch_get_dm", "dmx_batch_stats",
@@ -80,6 +80,7 @@ def load():
     lib.dmx_set_method.argtypes = [vp, C.c_int]
     lib.dmx_set_fusion_max_k.argtypes = [vp, C.c_uint32]
     lib.dmx_set_fusion_permute.argtypes = [vp, C.c_uint32]
+    lib.dmx_set_fusion_hermitian.argtypes = [vp, C.c_uint32]
     lib.dmx_last_run_stats.argtypes = [vp, P(C.c_double), P(C.c_uint32), P(C.c_uint32)]
     lib.dmx_last_run_fallbacks.argtypes = [vp, P(C.c_uint32)]
     lib.dmx_set_report.argtypes = [vp, C.c_int]
@@ -102,6 +103,7 @@ def load():
     lib.dmx_cost_set_params.argtypes = [vp, f32p, C.c_float, C.c_float]
     lib.dmx_set_cost_model.argtypes = [vp, vp]
     lib.dmx_set_tile_bits.argtypes = [vp, C.c_uint32]
+    lib.dmx_plan_sweeps.argtypes = [vp, P(C.c_uint32), P(C.c_uint32), P(C.c_uint32)]
     lib.dmx_plan_info.argtypes = [vp, P(C.c_uint32), P(C.c_uint32), P(C.c_uint32),
                                   P(C.c_double), P(C.c_double), P(C.c_uint32)]
     lib.dmx_time_launch.argtypes = [vp, C.c_int32, C.c_int32, C.c_uint32, C.c_int32, C.c_int32,
@@ -283,6 +285,11 @@ class Sim:
         permutation (default on; needs a second copy of rho)."""
         _check(load().dmx_set_fusion_permute(self._h, 1 if on else 0))
 
+    def set_fusion_hermitian(self, mode: int):
+        """Four-qubit blocks as Hermitian sweeps: 0 off, 1 from 12 qubits
+        (default), 2 from 8 qubits (needs permuting blocks on)."""
+        _check(load().dmx_set_fusion_hermitian(self._h, int(mode)))
+
     def set_profiling(self, on: bool):
         _check(load().dmx_set_profiling(self._h, 1 if on else 0))
 
@@ -339,7 +346,12 @@ class Sim:
                                     C.byref(pred), C.byref(greedy), tiles))
         return {"fused_blocks": fb.value, "plain_blocks": pb.value, "fused5_blocks": wg.value,
                 "predicted_us": pred.value, "greedy_us": greedy.value,
-                "tiles": {m: tiles[m - 4] for m in range(4, 9)}}
+                "tiles": {m: tiles[m - 4] for m in range(4, 9)}, **self.plan_sweeps()}
+
+    def plan_sweeps(self):
+        pb, hb, rl = C.c_uint32(), C.c_uint32(), C.c_uint32()
+        _check(load().dmx_plan_sweeps(self._h, C.byref(pb), C.byref(hb), C.byref(rl)))
+        return {"perm_blocks": pb.value, "herm_blocks": hb.value, "relabels": rl.value}
 
     def time_launch(self, kind, m=0, qmask=0, target=0, control=-1, slow=0, reps=10):
         """Average device microseconds of one launch configuration (rho is left

@@ -79,6 +79,14 @@ class GateTable:
         self.coefs.append(c)
         return self
 
+    def extend(self, other: "GateTable"):
+        """Append every gate of another table (same qubit count)."""
+        self.ops += list(other.ops)
+        self.qbs += [list(q) for q in other.qbs]
+        self.prms += [list(p) for p in other.prms]
+        self.coefs += [np.array(c) for c in other.coefs]
+        return self
+
     def add_matrix(self, qubits, mat):
         """Arbitrary 1- or 2-qubit matrix; qubits[0] is the high bit of the
         matrix index (C2_GATE convention: pos2 = base + 2^qubit1)."""

@@ -163,6 +163,15 @@ int dmx_set_fusion_max_k(dmx_sim* sim, uint32_t k);
  * from the next dmx_upload. */
 int dmx_set_fusion_permute(dmx_sim* sim, uint32_t on);
 
+/* Extension: with permuting blocks on, run four-qubit blocks as Hermitian
+ * sweeps (rho = rho^dagger throughout: only the tiles on and above the
+ * diagonal are read and transformed, each result is written with its
+ * conjugate transpose; rho's index bits are relabelled between blocks and
+ * restored at the end of the run).  mode 0 = off, 1 = from 12 qubits
+ * (default), 2 = from 8 qubits.  Out of place: uses the second pair of
+ * planes.  Used from the next dmx_upload. */
+int dmx_set_fusion_hermitian(dmx_sim* sim, uint32_t mode);
+
 /* Device time of the last run() in milliseconds (HIP events on the run's
  * stream), number of kernel launches it made, and number of full-matrix
  * sweeps it performed. */
@@ -295,6 +304,11 @@ int dmx_set_tile_bits(dmx_sim* sim, uint32_t m);
 int dmx_plan_info(const dmx_sim* sim, uint32_t* fused_blocks, uint32_t* plain_blocks,
                   uint32_t* fused5_blocks, double* predicted_us, double* greedy_us,
                   uint32_t* tiles);
+
+/* Extension: sweeps of the uploaded fused plan by kind -- four-qubit blocks
+ * with a trailing permutation, Hermitian block sweeps, relabelling sweeps. */
+int dmx_plan_sweeps(const dmx_sim* sim, uint32_t* perm_blocks, uint32_t* herm_blocks,
+                    uint32_t* relabels);
 
 /* Timing-data generator: average device time (microseconds) of reps
  * back-to-back launches of one kernel configuration (arguments as in

@@ -1,0 +1,280 @@
+// Host-side check of the Hermitian fused executor's index maps: plans built
+// with Hermitian sweeps (dmx::build_fused_plan, herm = true) are executed on
+// the CPU step by step with the same per-thread load/store maps the kernels
+// use (herm.hpp: herm_load/herm_direct/herm_mirror for k_fused4_herm,
+// relabel_load/relabel_store for k_relabel), on a random Hermitian stored
+// matrix S, and compared with S <- conj(M) S M^T applied gate by gate with the
+// gates' own matrices (dmx::gate_matrix).  Every store must stay inside the
+// planes, every output entry must be written exactly once per step, and the
+// final S must agree to 1e-12.  Prints "<cases> cases OK", or the first
+// violation and exits nonzero.
+#include <algorithm>
+#include <cmath>
+#include <complex>
+#include <cstdio>
+#include <random>
+#include <vector>
+
+#include "fuse.hpp"
+#include "herm.hpp"
+#include "lower.hpp"
+
+using dmx::Gate;
+using cd = std::complex<double>;
+
+static int fail(const char* what, int n, int seed)
+{
+    std::printf("FAIL %s (n=%d seed=%d)\n", what, n, seed);
+    return 1;
+}
+
+// Brick-wall layers of random two-qubit matrices, then one- and two-qubit
+// gates with many X, CX and SWAP (trailing index maps), then a few wider ones.
+static std::vector<Gate> circuit(int n, int seed)
+{
+    static const int kOps[] = {dmx::OP_U3, dmx::OP_RZ, dmx::OP_H, dmx::OP_X, dmx::OP_X,
+                               dmx::OP_CX, dmx::OP_CX, dmx::OP_CX, dmx::OP_SWAP, dmx::OP_SWAP,
+                               dmx::OP_CU1, dmx::OP_RZZ, dmx::OP_CCX, dmx::OP_RY};
+    constexpr int kNum = sizeof(kOps) / sizeof(kOps[0]);
+    std::mt19937 rng((uint32_t)seed);
+    std::uniform_real_distribution<double> ang(-3.2, 3.2);
+    std::normal_distribution<double> g(0.0, 1.0);
+    std::vector<Gate> c;
+    for (int layer = 0; layer < 3; layer++)
+        for (int q = layer & 1; q + 1 < n; q += 2) {
+            Gate m;
+            m.op = dmx::OP_MAT;
+            m.k = 2;
+            m.qb[0] = (uint32_t)q;
+            m.qb[1] = (uint32_t)q + 1;
+            m.mre.resize(16);
+            m.mim.resize(16);
+            // a unitary: exp(iH) is not needed, any 4x4 from Gram-Schmidt
+            std::vector<cd> M(16);
+            for (int col = 0; col < 4; col++) {
+                std::vector<cd> v(4);
+                for (int r = 0; r < 4; r++) v[r] = cd(g(rng), g(rng));
+                for (int p = 0; p < col; p++) {
+                    cd dot = 0.0;
+                    for (int r = 0; r < 4; r++) dot += std::conj(M[r * 4 + p]) * v[r];
+                    for (int r = 0; r < 4; r++) v[r] -= dot * M[r * 4 + p];
+                }
+                double nrm = 0;
+                for (int r = 0; r < 4; r++) nrm += std::norm(v[r]);
+                for (int r = 0; r < 4; r++) M[r * 4 + col] = v[r] / std::sqrt(nrm);
+            }
+            for (int i = 0; i < 16; i++) m.mre[i] = M[i].real(), m.mim[i] = M[i].imag();
+            c.push_back(m);
+        }
+    for (int i = 0; i < 90 + 10 * n; i++) {
+        Gate x;
+        x.op = kOps[rng() % kNum];
+        x.theta = ang(rng), x.phi = ang(rng), x.lambda = ang(rng);
+        std::vector<int> q(n);
+        for (int t = 0; t < n; t++) q[t] = t;
+        std::shuffle(q.begin(), q.end(), rng);
+        for (int t = 0; t < dmx::op_arity(x.op); t++) x.qb[t] = (uint32_t)q[t];
+        c.push_back(x);
+    }
+    return c;
+}
+
+// S <- conj(M) S M^T for one gate: rows of S take M, columns take conj(M).
+static void apply_gate(const Gate& g, int n, std::vector<cd>& S)
+{
+    int k = 0;
+    std::vector<uint32_t> qs;
+    std::vector<double> re, im;
+    dmx::gate_matrix(g, k, qs, re, im);
+    const int d = 1 << k;
+    const uint64_t dim = 1ull << n;
+    uint64_t gm = 0;
+    for (uint32_t q : qs) gm |= 1ull << q;
+    std::vector<uint64_t> idx(d);
+    std::vector<cd> in(d);
+    auto vec = [&](uint64_t stride, uint64_t offset, bool conj) {
+        for (uint64_t base = 0; base < dim; base++) {
+            if (base & gm) continue;
+            for (int l = 0; l < d; l++) {
+                uint64_t x = base;
+                for (int j = 0; j < k; j++)
+                    if ((l >> (k - 1 - j)) & 1) x |= 1ull << qs[j];
+                idx[l] = x;
+                in[l] = S[offset + x * stride];
+            }
+            for (int r = 0; r < d; r++) {
+                cd s = 0.0;
+                for (int l = 0; l < d; l++) {
+                    cd m(re[(size_t)r * d + l], im[(size_t)r * d + l]);
+                    s += (conj ? std::conj(m) : m) * in[l];
+                }
+                S[offset + idx[r] * stride] = s;
+            }
+        }
+    };
+    for (uint64_t r = 0; r < dim; r++) vec(1, r * dim, false);   // each row: M
+    for (uint64_t c = 0; c < dim; c++) vec(dim, c, true);        // each column: conj(M)
+}
+
+// One k_fused4_herm launch, thread by thread.
+static bool run_herm(const dmx::FusedBlock& blk, int n, const std::vector<cd>& in,
+                     std::vector<cd>& out, const char** why)
+{
+    dmx::HermGeom g;
+    g.init(blk, n);
+    const uint64_t num = 1ull << (2 * n);
+    std::vector<int> hits(num, 0);
+    std::vector<cd> lds(16 * dmx::kHermPitch);
+    // the launch's schedule (herm_slot) takes every item exactly once
+    for (uint32_t grid : {128u, 256u, 4096u}) {
+        std::vector<int> seen(g.items(), 0);
+        for (uint32_t b = 0; b < grid; b++) {
+            uint64_t w = 0;
+            for (uint64_t k = 0; dmx::herm_slot(g, b, grid, k, w); k++) {
+                if (w >= g.items()) return *why = "schedule item out of range", false;
+                seen[w]++;
+            }
+        }
+        for (int v : seen)
+            if (v != 1) return *why = "schedule takes an item other than once", false;
+    }
+    std::vector<dmx::HermThread> th(256);
+    for (int tid = 0; tid < 256; tid++) th[tid].init(g, tid);
+    for (uint64_t w = 0; w < g.items(); w++) {
+        const dmx::HermItem it = dmx::herm_item(g, w);
+        for (int tid = 0; tid < 256; tid++)
+            for (int rep = 0; rep < 8; rep++) {
+                int a;
+                const int lc = th[tid].lc;
+                const uint64_t off = dmx::herm_load(g, th[tid], it, rep, a);
+                if (off + 1 >= num) return *why = "load out of range", false;
+                lds[a * dmx::kHermPitch + lc] = in[off];
+                lds[a * dmx::kHermPitch + lc + 1] = in[off + 1];
+            }
+        for (int tb = 0; tb < 16; tb++) {
+            if (!dmx::herm_tile_on(it, tb)) continue;
+            cd T[16][16], X[16][16];
+            for (int a = 0; a < 16; a++)
+                for (int j = 0; j < 16; j++) T[a][j] = lds[a * dmx::kHermPitch + g.lds_col(tb, j)];
+            for (int a = 0; a < 16; a++)  // X = T U^T
+                for (int j = 0; j < 16; j++) {
+                    cd s = 0.0;
+                    for (int k = 0; k < 16; k++) s += T[a][k] * cd(blk.ur[j * 16 + k], blk.ui[j * 16 + k]);
+                    X[a][j] = s;
+                }
+            for (int a = 0; a < 16; a++)  // T' = conj(U) X
+                for (int j = 0; j < 16; j++) {
+                    cd s = 0.0;
+                    for (int k = 0; k < 16; k++) s += cd(blk.ur[a * 16 + k], -blk.ui[a * 16 + k]) * X[k][j];
+                    lds[a * dmx::kHermPitch + g.lds_col(tb, j)] = s;
+                }
+        }
+        for (int tid = 0; tid < 256; tid++)
+            for (int rep = 0; rep < 8; rep++) {
+                uint64_t off;
+                int a, c0, c1, a0, a1, col;
+                if (dmx::herm_direct(g, th[tid], it, rep, off, a, c0, c1)) {
+                    if (off + 1 >= num || (off & 1)) return *why = "direct store out of range", false;
+                    out[off] = lds[a * dmx::kHermPitch + c0];
+                    out[off + 1] = lds[a * dmx::kHermPitch + c1];
+                    hits[off]++, hits[off + 1]++;
+                }
+                if (dmx::herm_mirror(g, th[tid], it, rep, off, a0, a1, col)) {
+                    if (off + 1 >= num || (off & 1)) return *why = "mirror store out of range", false;
+                    out[off] = std::conj(lds[a0 * dmx::kHermPitch + col]);
+                    out[off + 1] = std::conj(lds[a1 * dmx::kHermPitch + col]);
+                    hits[off]++, hits[off + 1]++;
+                }
+            }
+    }
+    for (uint64_t i = 0; i < num; i++)
+        if (hits[i] != 1) return *why = "entry not written exactly once", false;
+    return true;
+}
+
+// One k_relabel launch, thread by thread.
+static bool run_relabel(const dmx::FusedBlock& blk, int n, const std::vector<cd>& in,
+                        std::vector<cd>& out, const char** why)
+{
+    dmx::RelabelGeom g;
+    g.init(blk, n);
+    const uint64_t num = 1ull << (2 * n);
+    std::vector<int> hits(num, 0);
+    std::vector<cd> lds(16 * dmx::kHermPitch);
+    std::vector<dmx::RelabelThread> th(256);
+    for (int tid = 0; tid < 256; tid++) th[tid].init(g, tid);
+    for (uint64_t w = 0; w < g.items(); w++) {
+        const dmx::RelabelItem it = dmx::relabel_item(g, w);
+        for (int tid = 0; tid < 256; tid++)
+            for (int rep = 0; rep < 8; rep++) {
+                int a;
+                const int lc = th[tid].lc;
+                const uint64_t off = dmx::relabel_load(g, th[tid], it, tid, rep, a);
+                if (off + 1 >= num) return *why = "relabel load out of range", false;
+                lds[a * dmx::kHermPitch + lc] = in[off];
+                lds[a * dmx::kHermPitch + lc + 1] = in[off + 1];
+            }
+        for (int tid = 0; tid < 256; tid++)
+            for (int rep = 0; rep < 8; rep++) {
+                int a;
+                const uint64_t off = dmx::relabel_store(g, th[tid], it, tid, rep, a);
+                if (off + 1 >= num || (off & 1)) return *why = "relabel store out of range", false;
+                out[off] = lds[a * dmx::kHermPitch + th[tid].s0];
+                out[off + 1] = lds[a * dmx::kHermPitch + th[tid].s1];
+                hits[off]++, hits[off + 1]++;
+            }
+    }
+    for (uint64_t i = 0; i < num; i++)
+        if (hits[i] != 1) return *why = "relabel entry not written exactly once", false;
+    return true;
+}
+
+int main()
+{
+    int cases = 0;
+    for (int n = 8; n <= 10; n++)
+        for (int seed = 0; seed < (n == 8 ? 3 : n == 9 ? 2 : 1); seed++) {
+            const std::vector<Gate> c = circuit(n, 100 * n + seed);
+            dmx::FusedPlan* plan = nullptr;
+            dmx::PlanInfo info;
+            if (dmx::build_fused_plan(c, n, 4, nullptr, 0, true, &plan, &info, true) != 0 || !plan)
+                return fail("build_fused_plan", n, seed);
+            std::vector<dmx::PlanStepView> steps;
+            plan->describe(steps);
+            if (info.herm_blocks == 0 || info.relabels != 1 || info.perm_blocks == 0)
+                return fail("plan has no Hermitian sweeps, relabel or trailing maps", n, seed);
+            const uint64_t dim = 1ull << n;
+            std::mt19937 rng(17u + (uint32_t)seed);
+            std::normal_distribution<double> g(0.0, 1.0);
+            std::vector<cd> S(dim * dim);
+            for (uint64_t i = 0; i < dim; i++)
+                for (uint64_t j = i; j < dim; j++) {
+                    const cd v = i == j ? cd(g(rng), 0.0) : cd(g(rng), g(rng));
+                    S[i * dim + j] = v;
+                    S[j * dim + i] = std::conj(v);
+                }
+            std::vector<cd> want = S, cur = S, nxt(dim * dim);
+            for (const Gate& gt : c) apply_gate(gt, n, want);
+            for (const auto& st : steps) {
+                const char* why = "";
+                if (st.kind == dmx::PlanStepView::HERM) {
+                    if (!run_herm(*st.block, n, cur, nxt, &why)) return fail(why, n, seed);
+                } else if (st.kind == dmx::PlanStepView::RELABEL) {
+                    if (!run_relabel(*st.block, n, cur, nxt, &why)) return fail(why, n, seed);
+                } else {
+                    return fail("unexpected step kind in a Hermitian plan", n, seed);
+                }
+                cur.swap(nxt);
+            }
+            double err = 0;
+            for (uint64_t i = 0; i < dim * dim; i++) err = std::max(err, std::abs(cur[i] - want[i]));
+            if (!(err < 1e-12)) {
+                std::printf("max |d| = %g\n", err);
+                return fail("result differs from gate-by-gate application", n, seed);
+            }
+            delete plan;
+            cases++;
+        }
+    std::printf("%d cases OK\n", cases);
+    return 0;
+}

@@ -156,7 +156,7 @@ static void apply(const Gate& g, std::vector<cd>& psi)
 int main()
 {
     int cases = 0;
-    for (int perm = 0; perm < 2; perm++)
+    for (int perm = 0; perm < 3; perm++)  // in place, permuting, Hermitian-sweep rule
     for (int n = 4; n <= 12; n++)
         for (int max_k = 4; max_k <= 5; max_k++)
             for (int seed = 0; seed < 6; seed++) {
@@ -165,7 +165,8 @@ int main()
                                                            channels);
                 dmx::BlockPlan bp;
                 bool fusable = false;
-                if (dmx::plan_blocks(c, n, max_k, bp, fusable, perm == 1) != 0 || !fusable)
+                if (dmx::plan_blocks(c, n, max_k, bp, fusable, perm >= 1, perm == 2) != 0 ||
+                    !fusable)
                     return fail("plan_blocks", n, max_k, seed);
                 if (bp.dense.size() != bp.blocks.size()) return fail("dense counts", n, max_k, seed);
                 // execution sequence of circuit indices (channels included)
@@ -204,7 +205,15 @@ int main()
                         if ((fb.perm_on != 0) != (dense < (int)mem.size()) || (perm == 0 && fb.perm_on))
                             return fail("permutation flag", n, max_k, seed);
                         if (fb.perm_on) {
-                            if (fb.perm_L[0] != 1u) return fail("permutation moves bit 0", n, max_k, seed);
+                            if (perm == 1 && fb.perm_L[0] != 1u)
+                                return fail("permutation moves bit 0", n, max_k, seed);
+                            if (perm == 2) {  // the block's bits must map onto a set of bits
+                                uint32_t img = 0;
+                                for (uint32_t m = fb.qmask; m; m &= m - 1)
+                                    img |= fb.perm_L[__builtin_ctz(m)];
+                                if (__builtin_popcount(img) != 4)
+                                    return fail("Hermitian map of the block's bits", n, max_k, seed);
+                            }
                             std::mt19937 r2((uint32_t)(seed + 31 * n));
                             for (int trial = 0; trial < 64; trial++) {
                                 const uint32_t x = (uint32_t)r2() & ((1u << n) - 1);

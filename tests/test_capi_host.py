@@ -138,4 +138,26 @@ def test_planner_grouping_is_valid(tmp_path):
     r = subprocess.run(cmd, capture_output=True, text=True)
     assert r.returncode == 0, r.stderr
     r = subprocess.run([exe], capture_output=True, text=True, timeout=120)
-    assert r.returncode == 0 and r.stdout.strip() == "216 cases OK", r.stdout
+    assert r.returncode == 0 and r.stdout.strip() == "324 cases OK", r.stdout
+
+
+def test_hermitian_sweep_maps(tmp_path):
+    """The Hermitian fused executor's index maps, executed on the CPU thread by
+    thread with the kernels' own map functions (dm-sim_amd/csrc/herm.hpp) on
+    plans from dmx::build_fused_plan at n = 8..10: every load and store inside
+    the planes, every entry written once per sweep, and the result equal to
+    the circuit applied gate by gate (tests/cpp/herm_maps.cpp)."""
+    import shutil
+    import subprocess
+    hipcc = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
+    if not os.path.exists(hipcc):
+        pytest.skip("hipcc not found")
+    exe = os.path.join(str(tmp_path), "herm_maps")
+    pkg = os.path.join(ROOT, "dm-sim_amd")
+    cmd = [hipcc, "-std=c++17", "-O2", "-I", os.path.join(pkg, "csrc"), "-I",
+           os.path.join(ROOT, "include"), os.path.join(ROOT, "tests", "cpp", "herm_maps.cpp"),
+           "-L", pkg, "-ldmsim_amd", f"-Wl,-rpath,{pkg}", "-o", exe]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    r = subprocess.run([exe], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0 and r.stdout.strip() == "6 cases OK", r.stdout
