@@ -145,7 +145,8 @@ struct Plan : FusedPlan {
                     return DMX_EINVAL;
                 }
                 const bool h = st.type == ST_HERM;
-                prof.begin(h ? KC_FUSED4H : KC_RELABEL, h ? herm_bytes(nq) : sweep_bytes);
+                prof.begin(h ? KC_FUSED4H : KC_RELABEL,
+                           h ? herm_bytes(nq, blocks[st.index].next_q != 0) : sweep_bytes);
                 e = h ? launch_fused4_herm(re, im, re2, im2, d_blocks + st.index, nq, max_blocks,
                                            stream)
                       : launch_relabel(re, im, re2, im2, d_blocks + st.index, nq, max_blocks,
@@ -1145,6 +1146,10 @@ void to_hermitian(Plan& plan, int n, PlanInfo& info)
         sinv = soinv;
     }
     go_home();
+    // a Hermitian sweep followed by another writes only what that one reads
+    for (size_t i = 0; i + 1 < steps.size(); i++)
+        if (steps[i].type == ST_HERM && steps[i + 1].type == ST_HERM)
+            plan.blocks[steps[i].index].next_q = plan.blocks[steps[i + 1].index].qmask;
     plan.steps.swap(steps);
 }
 

@@ -22,11 +22,15 @@ struct PlanStepView {
 
 // Algorithmic bytes of one Hermitian block sweep (k_fused4_herm) on n >= 8
 // qubits: the super-tiles on and above the diagonal read (16 * H(H+1)/2 of
-// them, H = 2^(n-8), 64 KiB each), every entry written.
-inline double herm_bytes(int n)
+// them, H = 2^(n-8), 64 KiB each); written: every entry, or (half, when the
+// next step is another Hermitian sweep) the tiles on and above that sweep's
+// diagonal, T(T+1)/2 of the T^2 tiles (T = 2^(n-4)).
+inline double herm_bytes(int n, bool half)
 {
     const double H = (double)(1ull << (n - 8));
-    return 16.0 * H * (H + 1.0) / 2.0 * 65536.0 + 16.0 * (double)(1ull << (2 * n));
+    const double T = (double)(1ull << (n - 4));
+    const double rho = 16.0 * (double)(1ull << (2 * n));
+    return 16.0 * H * (H + 1.0) / 2.0 * 65536.0 + (half ? rho * (T + 1.0) / (2.0 * T) : rho);
 }
 
 struct FusedPlan {

@@ -20,6 +20,11 @@ struct FusedBlock {
     double ur[256];
     double ui[256];
     uint32_t perm_L[16];
+    // Hermitian sweeps only: the next Hermitian block's index bits in this
+    // block's output order (its sweep reads only the tiles on and above its
+    // diagonal, so only those entries are written), 0 = write every entry.
+    uint32_t next_q;
+    uint32_t pad;
 };
 
 // One circuit's rho and the block to apply to it (batched launches).

@@ -52,6 +52,14 @@ struct HermGeom {
     uint32_t L[16];
     uint32_t pb;
     uint64_t itab;     // f^-1, four bits per entry
+    // Entries the next Hermitian sweep reads: (x, y) with x & ~Q' <= y & ~Q'
+    // (Q' = next_q, its index bits; compressing away bits keeps this order),
+    // i.e. the tiles on and above its diagonal.  For an output tile at rows
+    // R0 ^ f(a), columns C0 ^ f(j) that comparison is decided by the high
+    // parts R0 & hi_keep vs C0 & hi_keep unless they are equal; then both
+    // the tile and its mirror are written.  half = false: write every entry.
+    bool half;
+    uint32_t hi_keep;  // bits >= 4 outside next_q
 
     __host__ __device__ void init(const FusedBlock& blk, int nq)
     {
@@ -75,6 +83,8 @@ struct HermGeom {
 #pragma unroll
         for (int q = 0; q < 16; q++) L[q] = blk.perm_L[q];
         pb = blk.perm_b;
+        half = blk.next_q != 0;
+        hi_keep = (uint32_t)(dim - 1) & ~15u & ~blk.next_q;
         itab = 0;
         for (uint32_t a = 0; a < 16; a++)
             itab |= (uint64_t)a << (4 * (lin((uint32_t)deposit_bits(a, qmask)) & 15u));
@@ -224,6 +234,7 @@ __host__ __device__ inline bool herm_direct(const HermGeom& g, const HermThread&
 {
     if (it.diag && th.mid < (int)it.r_l) return false;
     const uint32_t C0 = it.Cb ^ th.Ld;
+    if (g.half && (it.R0 & g.hi_keep) > (C0 & g.hi_keep)) return false;  // the next sweep reads the mirror
     const uint32_t tc = C0 & 15u;
     c0 = th.fd + th.dq(g.inv((uint32_t)th.k2 ^ tc));
     c1 = th.fd + th.dq(g.inv((uint32_t)(th.k2 + 1) ^ tc));
@@ -242,6 +253,7 @@ __host__ __device__ inline bool herm_mirror(const HermGeom& g, const HermThread&
 {
     if (it.diag && th.top + 2 * rep <= (int)it.r_l) return false;
     const uint32_t C0 = it.Cb ^ th.Lm[rep];
+    if (g.half && (C0 & g.hi_keep) > (it.R0 & g.hi_keep)) return false;  // the next sweep reads the tile
     col = th.fm[rep] + th.dq(g.inv((uint32_t)th.mid ^ (C0 & 15u)));
     a0 = (int)g.inv((uint32_t)th.k2 ^ (it.R0 & 15u));
     a1 = (int)g.inv((uint32_t)(th.k2 + 1) ^ (it.R0 & 15u));

@@ -187,8 +187,18 @@ static bool run_herm(const dmx::FusedBlock& blk, int n, const std::vector<cd>& i
                 }
             }
     }
-    for (uint64_t i = 0; i < num; i++)
-        if (hits[i] != 1) return *why = "entry not written exactly once", false;
+    // every entry the next step reads written once (all of them unless the
+    // next step is a Hermitian sweep, which reads (x, y) with x & ~Q' <= y & ~Q'),
+    // none twice; the others are left stale and poisoned, so a read of one
+    // shows in the result
+    const uint64_t dim = 1ull << n;
+    const uint64_t keep = (dim - 1) & ~(uint64_t)blk.next_q;
+    for (uint64_t i = 0; i < num; i++) {
+        const bool needed = blk.next_q == 0 || ((i / dim) & keep) <= ((i % dim) & keep);
+        if (hits[i] > 1 || (needed && hits[i] != 1))
+            return *why = "entry the next step reads not written exactly once", false;
+        if (!hits[i]) out[i] = cd(std::nan(""), std::nan(""));
+    }
     return true;
 }
 
@@ -243,6 +253,10 @@ int main()
             plan->describe(steps);
             if (info.herm_blocks == 0 || info.relabels != 1 || info.perm_blocks == 0)
                 return fail("plan has no Hermitian sweeps, relabel or trailing maps", n, seed);
+            int halves = 0;
+            for (const auto& st : steps) halves += st.kind == dmx::PlanStepView::HERM && st.block->next_q;
+            if (halves + 1 != (int)info.herm_blocks)
+                return fail("every Hermitian sweep but the last writes half", n, seed);
             const uint64_t dim = 1ull << n;
             std::mt19937 rng(17u + (uint32_t)seed);
             std::normal_distribution<double> g(0.0, 1.0);
@@ -267,8 +281,13 @@ int main()
                 cur.swap(nxt);
             }
             double err = 0;
-            for (uint64_t i = 0; i < dim * dim; i++) err = std::max(err, std::abs(cur[i] - want[i]));
-            if (!(err < 1e-12)) {
+            bool finite = true;
+            for (uint64_t i = 0; i < dim * dim; i++) {
+                const double d = std::abs(cur[i] - want[i]);
+                finite = finite && std::isfinite(d);
+                err = std::max(err, d);
+            }
+            if (!finite || !(err < 1e-12)) {
                 std::printf("max |d| = %g\n", err);
                 return fail("result differs from gate-by-gate application", n, seed);
             }
