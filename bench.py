@@ -30,10 +30,16 @@ sys.path.insert(0, os.path.join(ROOT, "dm-sim_amd"))
 METRIC = "ρ gate-updates/s at n=14 FP64: fused+batched vs plain gate-by-gate; HBM GB/s"
 HBM_PEAK_GBS = 8000.0
 # HBM bytes per launch over algorithmic bytes per launch, per fused sweep kernel,
-# from rocprofv3 FETCH_SIZE and WRITE_SIZE runs of this workload scaled by a copy
-# of known size (profiles/k_fused4_n14_hbm_counters.md).
-TRAFFIC_RATIO = {"k_fused4": 8.591e9 / 8.590e9, "k_fused4_perm": 8.591e9 / 8.590e9}
+# measured by rocprofv3 FETCH_SIZE and WRITE_SIZE runs of this workload scaled by
+# a copy of known size; a constant here, from the summaries named beside it.
+TRAFFIC_RATIO = {
+    "k_fused4_herm": (4.361e9 / 4.342e9, "profiles/k_fused4_herm_n14_hbm_counters.md"),
+    "k_relabel": (8.590e9 / 8.590e9, "profiles/k_fused4_herm_n14_hbm_counters.md"),
+    "k_fused4": (8.591e9 / 8.590e9, "profiles/k_fused4_n14_hbm_counters.md"),
+    "k_fused4_perm": (8.591e9 / 8.590e9, "profiles/k_fused4_n14_hbm_counters.md"),
+}
 METHODS = {"plain": 0, "plain_launch": 1, "fused": 2, "fused_learned": 2, "fused_learned5": 2}
+W_OP_C2 = 39  # the 4x4 matrix extension op (workloads.OP["C2"])
 
 
 def log(msg):
@@ -59,16 +65,29 @@ def timed_runs(sim, steps, warmup):
     return wall, dev_ms, sim.last_run_stats(), prof
 
 
-def cpu_baseline(table, n, gates=120):
-    """The oracle (C restatement of the reference CPU path) on a slice of the
-    same workload's U3/CX tail, from |0><0|, on this host's cores."""
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(table, n, stride=8):
+    """The oracle (C restatement of the reference CPU path, gate after gate
+    over all rows as src/dmsim_cpu_omp.hpp:367-447 runs it) on every
+    stride-th gate of the same workload -- SU(4) matrices and U3/CX in the
+    workload's proportions -- from |0><0|, on this host's cores."""
     from oracle.oracle import Oracle
 
     threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or min(os.cpu_count() or 1, 16)
     o = Oracle()
     o.set_threads(threads)
-    start = 130
-    sample = table.slice(start, start + gates)
+    sample = table.take(range(0, len(table), stride))
+    n_c2 = sum(1 for op in sample.ops if op == W_OP_C2)
     t0 = time.perf_counter()
     o.sim(n, sample.arrays())
     dt = time.perf_counter() - t0
@@ -76,9 +95,11 @@ def cpu_baseline(table, n, gates=120):
         "value": len(sample) / dt,
         "unit": "gate-updates/s",
         "cores": threads,
+        "cpu": cpu_model(),
         "kind": "port",
-        "sample": f"oracle sim() of gates {start}..{start + len(sample) - 1} (U3/CX tail) of the "
-                  f"n={n} workload from |0><0| with {threads} OpenMP threads: {dt:.1f} s",
+        "sample": f"oracle sim() of every {stride}th gate of the n={n} workload ({len(sample)} "
+                  f"gates: {n_c2} SU(4), {len(sample) - n_c2} U3/CX) from |0><0| with {threads} "
+                  f"OpenMP threads: {dt:.1f} s",
     }
 
 
@@ -120,6 +141,15 @@ def batched(steps, warmup, seed=4, n_circuits=256):
         if tuned:
             out["launches_per_run"] = st["launches"]
             out["tile_bits"] = {n: b.tile_bits(n) for n in range(6, 12)}
+            # the batched run's block sweeps: 32 * 4^n bytes each, over the
+            # run's device time (which also holds the reset and the energies)
+            out["roofline"] = {
+                "bound": "hbm", "achieved": st["sweep_bytes"] / (st["ms"] * 1e-3) / 1e9,
+                "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": st["sweep_bytes"] / (st["ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                "bytes_per_run": st["sweep_bytes"], "device_ms_per_run": st["ms"],
+                "note": "rho of circuits up to 11 qubits fits the 256 MB Infinity Cache",
+            }
         b.close()
     for mname in ("plain", "plain_launch", "fused"):
         sims = {}
@@ -182,6 +212,8 @@ def vqe_published(warmup=1, steps=5):
         s.close()
         ms.sort()
         out[f"{mname}_ms"] = ms[len(ms) // 2]
+    if model is not None:
+        model.close()
     out["speedup_vs_reference"] = out["reference_ms"] / min(
         v for k, v in out.items() if k.endswith("_ms") and k != "reference_ms")
     return out
@@ -212,6 +244,8 @@ def main():
     ap.add_argument("--no-published", action="store_true",
                     help="skip the VQE-UCCSD comparison with the reference's published time")
     ap.add_argument("--calibrate-copy", action="store_true")
+    ap.add_argument("--gates", default="",
+                    help="LO:HI -- run only gates LO..HI-1 of the workload (profiling runs)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
@@ -226,6 +260,9 @@ def main():
 
     n = args.n
     table = W.brickwall_su4(n=n, layers=20, tail=2000, seed=14)
+    if args.gates:
+        lo, hi = (int(v) for v in args.gates.split(":"))
+        table = table.slice(lo, hi)
     gates = len(table)
     num = 4 ** n
     results = {}
@@ -233,6 +270,7 @@ def main():
     for mname in args.methods.split(","):
         sim = dmsim_amd.Sim(n)
         sim.set_method(METHODS[mname])
+        model = None
         if mname.startswith("fused_learned"):
             import costdata
             model = costdata.default_model()
@@ -267,6 +305,8 @@ def main():
             log(f"[bench]   {k:24s} calls={v['calls']:6d} mean={v['ms'] / v['calls'] * 1e3:9.1f} us "
                 f"{v['bytes'] / (v['ms'] * 1e-3) / 1e9:8.1f} GB/s")
         sim.close()
+        if model is not None:
+            model.close()
 
     best = max(results, key=lambda k: results[k]["gate_updates_per_s"])
     r = results[best]
@@ -305,8 +345,10 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": (TRAFFIC_RATIO[kname] * per_launch_bytes if kname in TRAFFIC_RATIO
+            "traffic": (TRAFFIC_RATIO[kname][0] * per_launch_bytes if kname in TRAFFIC_RATIO
                         else None),
+            "traffic_source": (f"measured ratio from {TRAFFIC_RATIO[kname][1]} times this run's "
+                               f"algorithmic bytes" if kname in TRAFFIC_RATIO else None),
             "kernel": kname,
             "bytes_per_launch": per_launch_bytes,
             "mean_launch_us": mean_s * 1e6,

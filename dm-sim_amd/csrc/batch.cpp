@@ -67,6 +67,13 @@ uint64_t Batch::gates() const
     return g;
 }
 
+double Batch::sweep_bytes() const
+{
+    double b = 0;
+    for (const Circ& c : circs_) b += (double)c.n_blocks * 32.0 * (double)(1ull << (2 * c.n));
+    return b;
+}
+
 int Batch::add_circuit(uint32_t n, std::vector<Gate> gates, uint32_t* id)
 {
     if (uploaded_) {

@@ -62,6 +62,9 @@ public:
     uint32_t launches() const { return (uint32_t)launches_.size() + 3; }
     uint32_t circuits() const { return (uint32_t)circs_.size(); }
     uint64_t gates() const;
+    // Algorithmic bytes of one run's block sweeps: read + write of both planes
+    // (32 * 4^n bytes) per block of every circuit.
+    double sweep_bytes() const;
 
 private:
     struct Circ {

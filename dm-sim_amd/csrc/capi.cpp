@@ -715,6 +715,13 @@ int dmx_batch_stats(const dmx_batch* b, double* ms, uint32_t* launches, uint32_t
     return DMX_OK;
 }
 
+int dmx_batch_sweep_bytes(const dmx_batch* b, double* bytes)
+{
+    if (!b) return null_handle();
+    if (bytes) *bytes = b->b.sweep_bytes();
+    return DMX_OK;
+}
+
 const char* dmx_last_error(void) { return dmx::get_error(); }
 
 }  // extern "C"

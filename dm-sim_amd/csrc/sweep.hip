@@ -149,21 +149,34 @@ struct Range {
     int lane;
 };
 
-// b >= 7.  cbp = control bit in pair-index space (-1: none).
+// Set bit c of x after inserting it (x's bits >= c move up one).
+__device__ __forceinline__ uint64_t ins1(uint64_t x, int c) { return ins0(x, c) | (1ull << c); }
+
+// b >= 7.  cbp = control bit in pair-index space (-1: none).  Only the pairs
+// with the control bit set are visited: whole wave units where it lies above
+// a unit (cbp >= 7), lane by lane where a lane's two pairs share it
+// (1 <= cbp <= 6: runs of 2^cbp pairs); cbp = 0 splits every lane's pair of
+// pairs, so those lanes load both and store one.
 template <int K, bool CJ>
 __device__ void sweep_wide(double* __restrict__ re, double* __restrict__ im, const double* c,
                            int b, int cbp, uint64_t N, Range w)
 {
     const bool coarse = cbp >= 7;
-    const uint64_t units = coarse ? (N >> 9) : (N >> 8);
+    const bool sparse = cbp >= 1 && cbp < 7 && N >= 512;
+    const uint64_t units = (coarse || sparse) ? (N >> 9) : (N >> 8);
     const uint64_t off1 = 1ull << b;
     for (uint64_t v = w.first; v < units; v += w.stride) {
-        const uint64_t u = coarse ? (ins0(v, cbp - 7) | (1ull << (cbp - 7))) : v;
-        const uint64_t p0 = (u << 7) + 2 * (uint64_t)w.lane;
+        uint64_t p0;
+        if (coarse)
+            p0 = (ins1(v, cbp - 7) << 7) + 2 * (uint64_t)w.lane;
+        else if (sparse)
+            p0 = ins1((v << 7) + 2 * (uint64_t)w.lane, cbp);
+        else
+            p0 = (v << 7) + 2 * (uint64_t)w.lane;
         const uint64_t pos0 = ins0(p0, b);
         const uint64_t pos1 = pos0 + off1;
         bool wa = true, wb = true;
-        if (cbp >= 0 && !coarse) {
+        if (cbp >= 0 && !coarse && !sparse) {
             wa = (p0 >> cbp) & 1;
             wb = ((p0 + 1) >> cbp) & 1;
         }
@@ -190,32 +203,40 @@ __device__ void sweep_wide(double* __restrict__ re, double* __restrict__ im, con
     }
 }
 
-// b <= 6.  cb = control bit in linear-index space (-1: none).
+// b <= 6.  cb = control bit in linear-index space (-1: none).  Only entries
+// with the control bit set are visited where it lies above a unit (cb >= 8),
+// above a half unit (cb = 7) or, lane by lane, above a lane's entry pair
+// (1 <= cb <= 6: the lanes walk the set half, so a pair partner sits in lane
+// ^ 2^(bx-1) with bx the target bit once the control bit is taken out);
+// cb = 0 splits every lane's pair, so those lanes store one entry.
 template <int K, bool CJ>
 __device__ void sweep_narrow(double* __restrict__ re, double* __restrict__ im, const double* c,
                              int b, int cb, uint64_t N, Range w)
 {
     const bool coarse = cb >= 8;
-    const uint64_t units = coarse ? (N >> 9) : (N >> 8);
+    const bool sparse = cb >= 1 && cb < 7 && N >= 512;
+    const uint64_t units = (coarse || sparse) ? (N >> 9) : (N >> 8);
+    const int bx = sparse && cb < b ? b - 1 : b;
     const int lane = w.lane;
     for (uint64_t v = w.first; v < units; v += w.stride) {
-        const uint64_t u = coarse ? (ins0(v, cb - 8) | (1ull << (cb - 8))) : v;
+        const uint64_t u = coarse ? ins1(v, cb - 8) : v;
 #pragma unroll
         for (int half = 0; half < 2; half++) {
             if (cb == 7 && half == 0) continue;  // control bit 7 clear in this half
-            const uint64_t e = (u << 8) + (uint64_t)half * 128 + 2 * (uint64_t)lane;
+            const uint64_t x = (u << 8) + (uint64_t)half * 128 + 2 * (uint64_t)lane;
+            const uint64_t e = sparse ? ins1(x, cb) : x;
             const double2 r = ld2(re + e), i = ld2(im + e);
             double xr, xi, yr, yi;  // new values of the lane's .x and .y entries
             bool wx = true, wy = true;
-            if (b == 0) {
+            if (bx == 0) {
                 eval0<K, CJ>(c, r.x, i.x, r.y, i.y, xr, xi);
                 eval1<K, CJ>(c, r.x, i.x, r.y, i.y, yr, yi);
                 if constexpr (pos1_only(K)) wx = false;
             } else {
-                const int m = 1 << (b - 1);
+                const int m = 1 << (bx - 1);
                 const double prx = __shfl_xor(r.x, m), pry = __shfl_xor(r.y, m);
                 const double pix = __shfl_xor(i.x, m), piy = __shfl_xor(i.y, m);
-                if (((lane >> (b - 1)) & 1) == 0) {
+                if (((lane >> (bx - 1)) & 1) == 0) {
                     eval0<K, CJ>(c, r.x, i.x, prx, pix, xr, xi);
                     eval0<K, CJ>(c, r.y, i.y, pry, piy, yr, yi);
                     if constexpr (pos1_only(K)) wx = wy = false;
@@ -224,7 +245,7 @@ __device__ void sweep_narrow(double* __restrict__ re, double* __restrict__ im, c
                     eval1<K, CJ>(c, pry, piy, r.y, i.y, yr, yi);
                 }
             }
-            if (cb >= 0 && cb < 7) {
+            if (cb >= 0 && cb < 7 && !sparse) {
                 if (cb == 0) {
                     wx = false;
                 } else if (((lane >> (cb - 1)) & 1) == 0) {

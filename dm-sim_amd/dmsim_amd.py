@@ -38,6 +38,7 @@ EXPORTS = [
     "dmx_batch_create", "dmx_batch_destroy", "dmx_batch_add_circuit", "dmx_batch_add_pauli",
     "dmx_batch_upload", "dmx_batch_set_graph", "dmx_batch_set_autotune", "dmx_batch_tile_bits",
     "dmx_batch_run", "dmx_batch_energies", "dmx_batch_get_dm", "dmx_batch_stats",
+    "dmx_batch_sweep_bytes",
     "dmx_last_error",
 ]
 
@@ -120,6 +121,7 @@ def load():
     lib.dmx_batch_tile_bits.argtypes = [vp, C.c_uint32, P(C.c_uint32)]
     lib.dmx_batch_energies.argtypes = [vp, P(C.c_double)]
     lib.dmx_batch_get_dm.argtypes = [vp, C.c_uint32, P(C.c_double), P(C.c_double)]
+    lib.dmx_batch_sweep_bytes.argtypes = [vp, P(C.c_double)]
     lib.dmx_batch_stats.argtypes = [vp, P(C.c_double), P(C.c_uint32), P(C.c_uint32),
                                     P(C.c_uint64)]
     lib.dmx_last_error.restype = C.c_char_p
@@ -539,4 +541,7 @@ class Batch:
         ms, la, nc, ng = C.c_double(), C.c_uint32(), C.c_uint32(), C.c_uint64()
         _check(load().dmx_batch_stats(self._h, C.byref(ms), C.byref(la), C.byref(nc),
                                       C.byref(ng)))
-        return {"ms": ms.value, "launches": la.value, "circuits": nc.value, "gates": ng.value}
+        by = C.c_double()
+        _check(load().dmx_batch_sweep_bytes(self._h, C.byref(by)))
+        return {"ms": ms.value, "launches": la.value, "circuits": nc.value, "gates": ng.value,
+                "sweep_bytes": by.value}

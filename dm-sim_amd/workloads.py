@@ -119,6 +119,16 @@ class GateTable:
         return (np.ascontiguousarray(ops), np.ascontiguousarray(qb),
                 np.ascontiguousarray(prm), np.ascontiguousarray(coef))
 
+    def take(self, indices):
+        """A table of the gates at the given positions, in that order."""
+        t = GateTable(self.n_qubits)
+        idx = list(indices)
+        t.ops = [self.ops[i] for i in idx]
+        t.qbs = [self.qbs[i] for i in idx]
+        t.prms = [self.prms[i] for i in idx]
+        t.coefs = [self.coefs[i] for i in idx]
+        return t
+
     def slice(self, lo, hi):
         t = GateTable(self.n_qubits)
         t.ops, t.qbs = self.ops[lo:hi], self.qbs[lo:hi]

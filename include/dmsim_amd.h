@@ -362,6 +362,11 @@ int dmx_batch_get_dm(dmx_batch* b, uint32_t id, double* re, double* im);
 int dmx_batch_stats(const dmx_batch* b, double* ms, uint32_t* launches, uint32_t* circuits,
                     uint64_t* gates);
 
+/* Extension: algorithmic bytes of one batched run's block sweeps (read and
+ * write of both planes, 32 * 4^n bytes, per block of every circuit; set at
+ * dmx_batch_upload). */
+int dmx_batch_sweep_bytes(const dmx_batch* b, double* bytes);
+
 /* Message of the last failure on this thread ("" if none). */
 const char* dmx_last_error(void);
 
