@@ -44,6 +44,13 @@ def launch_configs(n, rng, fused_sets=10):
         for qmask in sorted(sets):
             for m in range(5 if qmask & 1 else 6, min(8, n) + 1):  # m = 5 needs qubit 0
                 out.append((dm.COST_FUSED5, m, qmask, 0, -1, 0))
+    if n >= 8:  # Hermitian block sweeps (tile width fixed at 8)
+        sets = {0xF, ((1 << 4) - 1) << (n - 4), 0b11 | (0b11 << (n - 2))}
+        while len(sets) < min(fused_sets, math.comb(n, 4)):
+            q = rng.choice(n, 4, replace=False)
+            sets.add(int(sum(1 << int(x) for x in q)))
+        for qmask in sorted(sets):
+            out.append((dm.COST_HERM, 8, qmask, 0, -1, 0))
     for slow in (0, 1):
         for t in range(n):
             out.append((dm.COST_DENSE, 0, 0, t, -1, slow))
@@ -61,6 +68,8 @@ def est_us(n, kind):
     by = full / 2 if kind in (dm.COST_DIAG, dm.COST_CX) else full
     if kind == dm.COST_FUSED5:
         by *= 2  # MFMA-heavier than one streaming pass
+    if kind == dm.COST_HERM:
+        by *= 0.55  # the upper tiles read, half the entries written
     return 4.0 + by / 4.0e6
 
 

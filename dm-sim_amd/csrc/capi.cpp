@@ -315,6 +315,23 @@ int dmx_set_fusion_permute(dmx_sim* sim, uint32_t on)
     return guarded([&] { return sim->eng.set_fusion_permute(on != 0); });
 }
 
+int dmx_set_fusion_autotune(dmx_sim* sim, uint32_t on)
+{
+    if (!sim) return null_handle();
+    if (on > 1) {
+        dmx::set_error("fusion autotune flag must be 0 or 1");
+        return DMX_EINVAL;
+    }
+    return guarded([&] { return sim->eng.set_fusion_autotune(on != 0); });
+}
+
+int dmx_tuned_tile_bits(const dmx_sim* sim, uint32_t* m)
+{
+    if (!sim) return null_handle();
+    if (m) *m = (uint32_t)sim->eng.tuned_tile_bits();
+    return DMX_OK;
+}
+
 int dmx_set_fusion_hermitian(dmx_sim* sim, uint32_t mode)
 {
     if (!sim) return null_handle();
@@ -460,7 +477,7 @@ int dmx_cost_destroy(dmx_cost* model)
 int dmx_cost_features(uint32_t n_qubits, int32_t kind, int32_t m, uint32_t qmask, int32_t target,
                       int32_t control, int32_t slow, float* out)
 {
-    if (!out || n_qubits < 1 || n_qubits > 16 || kind < 0 || kind > DMX_COST_FUSED5) {
+    if (!out || n_qubits < 1 || n_qubits > 16 || kind < 0 || kind > DMX_COST_HERM) {
         dmx::set_error("cost features need an output, 1 <= n <= 16 and a known kind");
         return DMX_EINVAL;
     }
@@ -477,6 +494,11 @@ int dmx_cost_features(uint32_t n_qubits, int32_t kind, int32_t m, uint32_t qmask
             !dmx::tile_bits5_ok(n, m, qmask)) {
             dmx::set_error("five-qubit features need n >= 5, five qubits < n and "
                            "5 <= m <= min(8, n) (m = 5 only with qubit 0 in the block)");
+            return DMX_EINVAL;
+        }
+    } else if (kind == DMX_COST_HERM) {
+        if (n < 8 || __builtin_popcount(qmask) != 4 || (qmask >> n) != 0 || m != 8) {
+            dmx::set_error("Hermitian-sweep features need n >= 8, four qubits < n and m = 8");
             return DMX_EINVAL;
         }
     } else {
@@ -582,13 +604,14 @@ int dmx_plan_info(const dmx_sim* sim, uint32_t* fused_blocks, uint32_t* plain_bl
 }
 
 int dmx_plan_sweeps(const dmx_sim* sim, uint32_t* perm_blocks, uint32_t* herm_blocks,
-                    uint32_t* relabels)
+                    uint32_t* relabels, uint32_t* grouping)
 {
     if (!sim) return null_handle();
     const dmx::PlanInfo& pi = sim->eng.plan_info();
     if (perm_blocks) *perm_blocks = pi.perm_blocks;
     if (herm_blocks) *herm_blocks = pi.herm_blocks;
     if (relabels) *relabels = pi.relabels;
+    if (grouping) *grouping = pi.grouping;
     return DMX_OK;
 }
 

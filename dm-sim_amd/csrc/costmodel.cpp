@@ -28,16 +28,22 @@ void cost_features(int n, int kind, int m, uint32_t qmask, int target, int contr
     out[1 + kind] = 1.f;
     const uint64_t N = 1ull << (2 * n);
     uint64_t groups;
-    if (kind == CK_FUSED || kind == CK_FUSED5) {
-        const int k = kind == CK_FUSED ? 4 : 5;
-        out[7] = m / 8.f;
-        groups = std::min<uint64_t>(1ull << ((n - k) + (n - m)), kFusedGridCap);
-        out[9] = __builtin_popcount(qmask & 0xFu) / 4.f;
-        out[10] = (31 - __builtin_clz(qmask)) / 16.f;
-        out[11] = __builtin_ctz(qmask) / 16.f;
+    if (kind == CK_FUSED || kind == CK_FUSED5 || kind == CK_HERM) {
+        const int k = kind == CK_FUSED5 ? 5 : 4;
+        out[8] = m / 8.f;
+        if (kind == CK_HERM) {  // super-tiles on and above the diagonal, a multiple of 128
+            const uint64_t H = 1ull << (n - 8);
+            const uint64_t items = 16 * (H * (H + 1) / 2);
+            groups = (std::min<uint64_t>(items, kFusedGridCap) + 127) / 128 * 128;
+        } else {
+            groups = std::min<uint64_t>(1ull << ((n - k) + (n - m)), kFusedGridCap);
+        }
+        out[10] = __builtin_popcount(qmask & 0xFu) / 4.f;
+        out[11] = (31 - __builtin_clz(qmask)) / 16.f;
+        out[12] = __builtin_ctz(qmask) / 16.f;
         uint32_t cmask = qmask;  // super-tile column bits, as the fused kernels form them
         for (int q = 0; q < n && __builtin_popcount(cmask) < m; q++) cmask |= 1u << q;
-        out[12] = __builtin_ctz(~cmask) / 8.f;
+        out[13] = __builtin_ctz(~cmask) / 8.f;
     } else {
         const int shift = slow ? n : 0;
         const int b = target + shift;
@@ -45,12 +51,12 @@ void cost_features(int n, int kind, int m, uint32_t qmask, int target, int contr
             groups = std::min<uint64_t>((N / 4 + 255) / 256, kSweepGridCap);
         else
             groups = N < 256 ? 1 : std::min<uint64_t>((N / 256 + 3) / 4, kSweepGridCap);
-        out[9] = b < 7 ? 1.f : 0.f;
-        out[10] = b / 32.f;
-        out[11] = (kind == CK_QUAD ? control + shift : b) / 32.f;
-        if (kind == CK_CX) out[12] = (control + shift + 1) / 32.f;
+        out[10] = b < 7 ? 1.f : 0.f;
+        out[11] = b / 32.f;
+        out[12] = (kind == CK_QUAD ? control + shift : b) / 32.f;
+        if (kind == CK_CX) out[13] = (control + shift + 1) / 32.f;
     }
-    out[8] = (float)std::log2((double)groups) / 16.f;
+    out[9] = (float)std::log2((double)groups) / 16.f;
 }
 
 CostModel::~CostModel()

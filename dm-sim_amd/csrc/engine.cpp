@@ -202,6 +202,27 @@ int Engine::upload()
         if (plan_) {
             rc = plan_->upload(stream_);
             if (rc) return rc;
+            tuned_tile_bits_ = 0;
+            if (!model_ && tile_bits_ == 0 && fusion_autotune_ && plan_info_.herm_blocks == 0) {
+                // the greedy plan's tile width, measured: on the second pair of
+                // planes if there is one, else on scratch planes (rho untouched)
+                double *sr = d_re2_, *si = d_im2_;
+                const bool temp = !sr;
+                if (temp) {
+                    HIPTRY(hipMalloc(&sr, num_ * sizeof(double)));
+                    const hipError_t e = hipMalloc(&si, num_ * sizeof(double));
+                    if (e != hipSuccess) {
+                        (void)hipFree(sr);
+                        HIPTRY(e);
+                    }
+                }
+                rc = plan_->autotune_tiles(n_, sr, si, stream_, &tuned_tile_bits_, plan_info_);
+                if (temp) {
+                    (void)hipFree(sr);
+                    (void)hipFree(si);
+                }
+                if (rc) return rc;
+            }
         } else if (wide) {
             set_error("matrix gates on 3..5 qubits need a fusable circuit (n >= 4, no SRN)");
             return DMX_ESTATE;
@@ -614,6 +635,22 @@ int Engine::time_launch(int kind, int m, uint32_t qmask, int target, int control
                       "(m = 5 only with qubit 0 in the block)");
             return DMX_EINVAL;
         }
+    } else if (kind == CK_HERM) {
+        if (n_ < 8 || __builtin_popcount(qmask) != 4 || (qmask >> n_) != 0 || m != 8) {
+            set_error("Hermitian-sweep timing needs n >= 8, four qubits < n and m = 8");
+            return DMX_EINVAL;
+        }
+        if (!d_re2_) {  // the sweep writes the second pair of planes
+            double *re2 = nullptr, *im2 = nullptr;
+            HIPTRY(hipMalloc(&re2, num_ * sizeof(double)));
+            hipError_t e = hipMalloc(&im2, num_ * sizeof(double));
+            if (e != hipSuccess) {
+                (void)hipFree(re2);
+                HIPTRY(e);
+            }
+            d_re2_ = re2;
+            d_im2_ = im2;
+        }
     } else if (kind >= CK_DENSE && kind <= CK_QUAD) {
         const bool two = kind == CK_CX || kind == CK_QUAD;
         if (target < 0 || target >= n_ || (two && (control < 0 || control >= n_ || control == target))) {
@@ -660,11 +697,25 @@ int Engine::time_launch(int kind, int m, uint32_t qmask, int target, int control
                 fb[0].ui[32 * r + c] = s * std::sin(2 * pi * r * c / 32);
             }
         HIPTRY(hipMemcpy(d_probe_blk5_, fb.data(), sizeof(FusedBlock5), hipMemcpyHostToDevice));
-    } else if (kind == CK_FUSED) {  // 16 x 16 DFT / 4
+    } else if (kind == CK_FUSED || kind == CK_HERM) {  // 16 x 16 DFT / 4
         FusedBlock fb;
         std::memset(&fb, 0, sizeof(fb));
         fb.qmask = qmask;
         fb.m = (uint32_t)m;
+        if (kind == CK_HERM) {
+            // the output order a plan gives it: the block's qubits at bits 0..3,
+            // the super-tile's four low free bits at 4..7, the rest in order;
+            // the next sweep's bits at the top (so half the entries are written)
+            uint32_t cm = qmask;
+            for (int q = 0; q < n_ && __builtin_popcount(cm) < 8; q++) cm |= 1u << q;
+            int next = 0;
+            for (uint32_t b = qmask; b; b &= b - 1) fb.perm_L[__builtin_ctz(b)] = 1u << next++;
+            for (uint32_t b = cm & ~qmask; b; b &= b - 1) fb.perm_L[__builtin_ctz(b)] = 1u << next++;
+            for (int q = 0; q < n_; q++)
+                if (!((cm >> q) & 1u)) fb.perm_L[q] = 1u << next++;
+            fb.perm_on = 1;
+            fb.next_q = 0xFu << (n_ - 4);
+        }
         for (int r = 0; r < 16; r++)
             for (int c = 0; c < 16; c++) {
                 fb.ur[16 * r + c] = 0.25 * std::cos(2 * pi * r * c / 16);
@@ -677,6 +728,9 @@ int Engine::time_launch(int kind, int m, uint32_t qmask, int target, int control
     auto launch = [&]() {
         if (kind == CK_FUSED5)
             return launch_fused5(d_re_, d_im_, d_probe_blk5_, n_, m, kFusedGridCap, stream_);
+        if (kind == CK_HERM)
+            return launch_fused4_herm(d_re_, d_im_, d_re2_, d_im2_, d_probe_blk_, n_,
+                                      kFusedGridCap, stream_);
         return kind == CK_FUSED
                    ? launch_fused4(d_re_, d_im_, d_probe_blk_, n_, m, kFusedGridCap, stream_)
                    : launch_plain_one(d_re_, d_im_, d_probe_op_, op, n_, slow != 0, kSweepGridCap,

@@ -51,6 +51,15 @@ public:
         fusion_herm_ = mode;
         return 0;
     }
+    // Without a cost model, below the Hermitian-sweep size: time the plan's
+    // block sweeps at every tile width at upload and keep the fastest (on by
+    // default).  Takes effect at the next upload().
+    int set_fusion_autotune(bool on)
+    {
+        fusion_autotune_ = on;
+        return 0;
+    }
+    int tuned_tile_bits() const { return tuned_tile_bits_; }
     // Cost model for the fused planner (not owned; nullptr = greedy plan).
     // Takes effect at the next upload().
     void set_cost_model(CostModel* model) { model_ = model; }
@@ -109,6 +118,8 @@ private:
     int fusion_max_k_ = 4;
     bool fusion_permute_ = true;
     int fusion_herm_ = 1;
+    bool fusion_autotune_ = true;
+    int tuned_tile_bits_ = 0;  // width the last upload's timing chose (0: none)
     bool herm_wanted() const;
     bool permuting_pays(const PlanInfo& perm, const PlanInfo& inplace) const;
     int persistent_blocks_ = 0;

@@ -106,6 +106,54 @@ struct Plan : FusedPlan {
         return false;
     }
 
+    int autotune_tiles(int nq, double* re, double* im, hipStream_t stream, int* chosen,
+                       PlanInfo& info) override
+    {
+        *chosen = 0;
+        std::vector<int> sample;
+        for (const Step& st : steps)
+            if (st.type == ST_FUSED4 && sample.size() < 8) sample.push_back(st.index);
+        if (sample.empty()) return DMX_OK;
+        bool all_q0 = true;
+        for (int b : sample) all_q0 = all_q0 && (blocks[b].qmask & 1u);
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        hipError_t e = hipEventCreate(&e0);
+        if (e == hipSuccess) e = hipEventCreate(&e1);
+        int best_m = 0;
+        float best_ms = 0.f;
+        for (int m = all_q0 ? 4 : 5; m <= std::min(8, nq) && e == hipSuccess; m++) {
+            for (int rep = 0; rep < 4 && e == hipSuccess; rep++) {  // rep 0 warms up
+                if (rep == 1) e = hipEventRecord(e0, stream);
+                for (int b : sample)
+                    if (e == hipSuccess)
+                        e = launch_fused4(re, im, d_blocks + b, nq, m, max_blocks, stream);
+            }
+            if (e == hipSuccess) e = hipEventRecord(e1, stream);
+            if (e == hipSuccess) e = hipEventSynchronize(e1);
+            float ms = 0.f;
+            if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+            if (e == hipSuccess && (best_m == 0 || ms < best_ms)) {
+                best_m = m;
+                best_ms = ms;
+            }
+        }
+        if (e0) (void)hipEventDestroy(e0);
+        if (e1) (void)hipEventDestroy(e1);
+        if (e != hipSuccess) {
+            set_error(std::string("tile-width timing: ") + hipGetErrorString(e));
+            return DMX_EHIP;
+        }
+        for (int m = 4; m <= 8; m++) info.tile_hist[m] = 0;
+        for (const Step& st : steps)
+            if (st.type == ST_FUSED4) {
+                FusedBlock& fb = blocks[st.index];
+                if (tile_bits_ok(nq, best_m, fb.qmask)) fb.m = (uint32_t)best_m;
+                info.tile_hist[fb.m]++;
+            }
+        *chosen = best_m;
+        return DMX_OK;
+    }
+
     void describe(std::vector<PlanStepView>& out) const override
     {
         out.clear();
@@ -498,7 +546,18 @@ private:
 
     bool taken(int gi) const { return done_[gi] || mark_[gi] == stamp_; }
 
-    long weight(int gi) const { return __builtin_popcount(mask_[gi]) > 1 ? 100 : 1; }
+    long weight(int gi) const
+    {
+        if (weights_) return (*weights_)[gi];
+        return __builtin_popcount(mask_[gi]) > 1 ? 100 : 1;
+    }
+
+public:
+    // Gate weights for the candidates' scores (circuit index -> weight); the
+    // default is 100 per multi-qubit gate and 1 per one-qubit gate.
+    void set_weights(const std::vector<long>* w) { weights_ = w; }
+
+private:
 
     // Whether op's matrix (operands on qubits 0, 1) is exactly the permutation
     // local index l -> f(l).
@@ -763,12 +822,14 @@ private:
     // op and angles (matrix gates are not cached)
     std::map<std::tuple<int, double, double, double>, std::pair<uint32_t, uint32_t>> class_cache_;
     bool perm_op_[OP_COUNT] = {};  // ops that may join a block's index map
+    const std::vector<long>* weights_ = nullptr;
 };
 
 void group_frontier(const std::vector<Gate>& circuit, int n, int kmax, bool allow_perm, bool herm,
-                    std::vector<PB>& pbs, Seq& seq)
+                    std::vector<PB>& pbs, Seq& seq, const std::vector<long>* weights = nullptr)
 {
     FrontierGrouper fg(circuit, n, kmax, allow_perm, herm);
+    fg.set_weights(weights);
     std::vector<int> seg;
     for (int gi = 0; gi < (int)circuit.size(); gi++) {
         const Gate& g = circuit[gi];
@@ -784,6 +845,11 @@ void group_frontier(const std::vector<Gate>& circuit, int n, int kmax, bool allo
     fg.run(seg, pbs, seq);
 }
 
+// With a cost model: the grouping the network prices cheapest (defined below
+// with the pricing helpers).
+int choose_grouping(const std::vector<Gate>& circuit, int n, int kmax, bool allow_perm, bool herm,
+                    CostModel& model, std::vector<PB>& pbs, Seq& seq, uint32_t& which);
+
 // Sweeps a grouping costs: a five-qubit block counts as about 1.5 four-qubit
 // sweeps (its MFMA work is eight times a four-qubit block's per entry).
 double grouping_cost(const std::vector<PB>& pbs)
@@ -796,7 +862,7 @@ double grouping_cost(const std::vector<PB>& pbs)
 }  // namespace
 
 int plan_blocks(const std::vector<Gate>& circuit, int n, int max_k, BlockPlan& out, bool& fusable,
-                bool allow_perm, bool herm)
+                bool allow_perm, bool herm, CostModel* model, uint32_t* grouping)
 {
     out = BlockPlan{};
     fusable = false;
@@ -805,15 +871,24 @@ int plan_blocks(const std::vector<Gate>& circuit, int n, int max_k, BlockPlan& o
     for (const Gate& g : circuit)
         if (g.op == OP_SRN) return DMX_OK;  // not complex-linear: plain sweeps
     fusable = true;
-    // both groupings; the one with fewer sweeps is kept (the first on ties)
+    // without a model: both groupings, the one with fewer sweeps kept (the
+    // first on ties); with one: the cheapest of three as the network prices them
     std::vector<PB> pbs, pbs2;
     Seq seq, seq2;
-    group_least_growth(circuit, kmax, pbs, seq);
-    group_frontier(circuit, n, kmax, allow_perm, herm, pbs2, seq2);
-    if (grouping_cost(pbs2) < grouping_cost(pbs)) {
-        pbs.swap(pbs2);
-        seq.swap(seq2);
+    uint32_t which = 0;
+    if (model) {
+        const int rc = choose_grouping(circuit, n, kmax, allow_perm, herm, *model, pbs, seq, which);
+        if (rc) return rc;
+    } else {
+        group_least_growth(circuit, kmax, pbs, seq);
+        group_frontier(circuit, n, kmax, allow_perm, herm, pbs2, seq2);
+        if (grouping_cost(pbs2) < grouping_cost(pbs)) {
+            pbs.swap(pbs2);
+            seq.swap(seq2);
+            which = 1;
+        }
     }
+    if (grouping) *grouping = which;
     const uint32_t all = (1u << n) - 1;
     std::vector<std::pair<int, int>> where(pbs.size());  // pbs index -> order entry
     for (size_t bi = 0; bi < pbs.size(); bi++) {
@@ -901,6 +976,117 @@ int cost_kind(const PrimOp& p)
     }
 }
 
+// Predicted time of each gate run unfused: its primitive sweeps on both
+// indices (us[gi]; a gate with no primitive sweeps -- a 3..5-qubit matrix --
+// gets a very large value, an identity or channel 0).
+int unfused_gate_times(const std::vector<Gate>& circuit, int n, CostModel& model,
+                       std::vector<double>& us)
+{
+    std::vector<std::vector<float>> feats;
+    std::vector<size_t> first(circuit.size() + 1, 0);
+    std::vector<char> none(circuit.size(), 0);
+    float f[kMlpFeatures];
+    for (size_t gi = 0; gi < circuit.size(); gi++) {
+        first[gi] = feats.size();
+        const Gate& g = circuit[gi];
+        if (g.op == OP_ID || g.op == OP_KRAUS) continue;
+        std::vector<PrimOp> ps;
+        if (!lower_gate(g, ps)) {
+            none[gi] = 1;
+            continue;
+        }
+        for (int side = 0; side < 2; side++)
+            for (const PrimOp& p : ps) {
+                cost_features(n, cost_kind(p), 0, 0, p.q0, p.q1, side, f);
+                feats.emplace_back(f, f + kMlpFeatures);
+            }
+    }
+    first[circuit.size()] = feats.size();
+    std::vector<float> t;
+    const int rc = model.price(feats, t);
+    if (rc) return rc;
+    us.assign(circuit.size(), 0.0);
+    for (size_t gi = 0; gi < circuit.size(); gi++) {
+        if (none[gi]) {
+            us[gi] = 1e12;
+            continue;
+        }
+        for (size_t i = first[gi]; i < first[gi + 1]; i++) us[gi] += t[i];
+    }
+    return DMX_OK;
+}
+
+// Predicted time of a grouping: per block the cheaper of its fused sweep (a
+// Hermitian sweep, or k_fused4 at the default width; k_fused5 for five-qubit
+// blocks) and its gates run unfused.
+int price_grouping(const std::vector<PB>& pbs, int n, bool herm, const std::vector<double>& gate_us,
+                   CostModel& model, double& total)
+{
+    std::vector<std::vector<float>> feats;
+    float f[kMlpFeatures];
+    for (const PB& pb : pbs) {
+        uint32_t q = pb.mask;
+        for (int b = 0; b < n && __builtin_popcount(q) < 4; b++) q |= 1u << b;
+        if (__builtin_popcount(q) == 5) {
+            int m = default_tile_bits5(n);
+            if (!tile_bits5_ok(n, m, q)) m = std::min(8, n);
+            cost_features(n, CK_FUSED5, m, q, 0, -1, 0, f);
+        } else if (herm) {
+            cost_features(n, CK_HERM, 8, q, 0, -1, 0, f);
+        } else {
+            cost_features(n, CK_FUSED, default_tile_bits(n), q, 0, -1, 0, f);
+        }
+        feats.emplace_back(f, f + kMlpFeatures);
+    }
+    std::vector<float> t;
+    const int rc = model.price(feats, t);
+    if (rc) return rc;
+    total = 0;
+    for (size_t b = 0; b < pbs.size(); b++) {
+        double unfused = 0;
+        for (int gi : pbs[b].gates) unfused += gate_us[gi];
+        total += __builtin_popcount(pbs[b].mask) > 4 ? (double)t[b] : std::min((double)t[b], unfused);
+    }
+    return DMX_OK;
+}
+
+}  // namespace
+
+namespace {
+
+int choose_grouping(const std::vector<Gate>& circuit, int n, int kmax, bool allow_perm, bool herm,
+                    CostModel& model, std::vector<PB>& pbs, Seq& seq, uint32_t& which)
+{
+    std::vector<double> gate_us;
+    int rc = unfused_gate_times(circuit, n, model, gate_us);
+    if (rc) return rc;
+    // third candidate: the frontier search scoring each block by the predicted
+    // unfused time of the gates it takes (hundredths of a microsecond)
+    std::vector<long> w(circuit.size(), 1);
+    for (size_t gi = 0; gi < circuit.size(); gi++)
+        w[gi] = std::max<long>(1, std::lround(std::min(gate_us[gi], 1e9) * 100.0));
+    std::vector<PB> cand[3];
+    Seq cseq[3];
+    group_least_growth(circuit, kmax, cand[0], cseq[0]);
+    group_frontier(circuit, n, kmax, allow_perm, herm, cand[1], cseq[1]);
+    group_frontier(circuit, n, kmax, allow_perm, herm, cand[2], cseq[2], &w);
+    double best = 0;
+    which = 0;
+    for (uint32_t c = 0; c < 3; c++) {
+        double t = 0;
+        rc = price_grouping(cand[c], n, herm, gate_us, model, t);
+        if (rc) return rc;
+        if (c == 0 || t < best ||
+            (t == best && grouping_cost(cand[c]) < grouping_cost(cand[which]))) {
+            best = t;
+            which = c;
+        }
+    }
+    pbs.swap(cand[which]);
+    seq.swap(cseq[which]);
+    return DMX_OK;
+}
+
 // Learned choice per block: fused at tile width m (4..min(8, n)) or its
 // member gates' primitive sweeps on both indices.  Sets blocks[b].m, fills
 // plain[b] with the primitives of blocks that run unfused.
@@ -909,10 +1095,11 @@ int cost_kind(const PrimOp& p)
 constexpr double kMargin = 0.92;
 
 int choose_executions(const std::vector<Gate>& circuit, int n, BlockPlan& bp, CostModel& model,
-                      std::vector<std::vector<PrimOp>>& plain, PlanInfo& info)
+                      std::vector<std::vector<PrimOp>>& plain, PlanInfo& info, bool herm)
 {
     const int nb = (int)bp.blocks.size();
-    const int m_hi = std::min(8, n);
+    // Hermitian plans: one fused option per block, the Hermitian sweep at width 8
+    const int m_hi = herm ? 4 : std::min(8, n);
     const int n_m = m_hi - 4 + 1;
     std::vector<std::vector<float>> feats;
     std::vector<size_t> first(nb);
@@ -922,9 +1109,13 @@ int choose_executions(const std::vector<Gate>& circuit, int n, BlockPlan& bp, Co
     for (int b = 0; b < nb; b++) {
         first[b] = feats.size();
         for (int m = 4; m <= m_hi; m++) {
-            // an inadmissible width is priced like a valid one and never chosen
-            const int mm = tile_bits_ok(n, m, bp.blocks[b].qmask) ? m : m_hi;
-            cost_features(n, CK_FUSED, mm, bp.blocks[b].qmask, 0, -1, 0, f);
+            if (herm) {
+                cost_features(n, CK_HERM, 8, bp.blocks[b].qmask, 0, -1, 0, f);
+            } else {
+                // an inadmissible width is priced like a valid one and never chosen
+                const int mm = tile_bits_ok(n, m, bp.blocks[b].qmask) ? m : m_hi;
+                cost_features(n, CK_FUSED, mm, bp.blocks[b].qmask, 0, -1, 0, f);
+            }
             feats.emplace_back(f, f + kMlpFeatures);
         }
         std::vector<PrimOp> ps;
@@ -942,18 +1133,19 @@ int choose_executions(const std::vector<Gate>& circuit, int n, BlockPlan& bp, Co
     std::vector<float> us;
     int rc = model.price(feats, us);
     if (rc) return rc;
-    const int m_default = default_tile_bits(n);
+    const int m_default = herm ? 4 : default_tile_bits(n);  // herm: the one option, slot 0
     for (int b = 0; b < nb; b++) {
         const float* t = us.data() + first[b];
-        // leave the default only for a predicted gain above the model's noise
+        // the width the network prices fastest (one kernel at different
+        // widths: no margin, its errors there are alike)
         int best_m = m_default;
         double best = t[m_default - 4];
-        for (int i = 0; i < n_m; i++)
-            if (tile_bits_ok(n, 4 + i, bp.blocks[b].qmask) && t[i] < kMargin * t[m_default - 4] &&
-                t[i] < best) {
+        for (int i = 0; i < n_m && !herm; i++)
+            if (tile_bits_ok(n, 4 + i, bp.blocks[b].qmask) && t[i] < best) {
                 best_m = 4 + i;
                 best = t[i];
             }
+        if (herm) best_m = 8;
         info.greedy_us += t[m_default - 4];
         bool use_plain = false;
         if (n_plain[b] >= 0) {
@@ -1010,8 +1202,7 @@ int choose_fused5(const std::vector<Gate>& circuit, int n, BlockPlan& bp, CostMo
         int best_m = md;
         double best = t[md - 5];
         for (int i = 0; i < n_m; i++)
-            if (tile_bits5_ok(n, 5 + i, bp.blocks5[b].qmask) && t[i] < kMargin * t[md - 5] &&
-                t[i] < best) {
+            if (tile_bits5_ok(n, 5 + i, bp.blocks5[b].qmask) && t[i] < best) {
                 best_m = 5 + i;
                 best = t[i];
             }
@@ -1025,7 +1216,7 @@ int choose_fused5(const std::vector<Gate>& circuit, int n, BlockPlan& bp, CostMo
         if (rc) return rc;
         if (fusable && sp.bp.blocks5.empty()) {  // a five-qubit gate cannot be split
             PlanInfo si;
-            rc = choose_executions(gates, n, sp.bp, model, sp.plain, si);
+            rc = choose_executions(gates, n, sp.bp, model, sp.plain, si, false);
             if (rc) return rc;
             if (si.predicted_us < kMargin * best) {
                 best = si.predicted_us;
@@ -1163,7 +1354,8 @@ int build_fused_plan(const std::vector<Gate>& circuit, int n, int max_k, CostMod
     BlockPlan bp;
     bool fusable = false;
     herm = herm && allow_perm && n >= kHermMinQubits && n <= 16;
-    int rc = plan_blocks(circuit, n, max_k, bp, fusable, allow_perm, herm);
+    int rc = plan_blocks(circuit, n, max_k, bp, fusable, allow_perm, herm,
+                         tile_bits > 0 ? nullptr : model, &pi.grouping);
     if (rc || !fusable) return rc;
     std::vector<std::vector<PrimOp>> plain(bp.blocks.size());
     std::vector<SubPlan> sub(bp.blocks5.size());
@@ -1173,7 +1365,7 @@ int build_fused_plan(const std::vector<Gate>& circuit, int n, int max_k, CostMod
         for (FusedBlock5& fb : bp.blocks5)
             if (tile_bits5_ok(n, tile_bits, fb.qmask)) fb.m = (uint32_t)tile_bits;
     } else if (model) {
-        rc = choose_executions(circuit, n, bp, *model, plain, pi);
+        rc = choose_executions(circuit, n, bp, *model, plain, pi, herm);
         if (rc) return rc;
         rc = choose_fused5(circuit, n, bp, *model, sub, pi);
         if (rc) return rc;

@@ -13,6 +13,26 @@
 
 namespace dmx {
 
+class CostModel;  // mlp.hpp
+
+// What the planner chose.  predicted_us is the cost model's estimate of the
+// run's launches (0 without a model); greedy_us its estimate for every block
+// fused at the default tile width.
+struct PlanInfo {
+    uint32_t fused_blocks = 0;   // four-qubit blocks run fused
+    uint32_t plain_blocks = 0;   // four-qubit blocks run as primitive sweeps on both indices
+    uint32_t fused5_blocks = 0;  // five-qubit blocks run fused (a split one counts as its
+                                 // four-qubit sub-blocks instead)
+    uint32_t channels = 0;       // noise channels (one superoperator sweep each)
+    uint32_t perm_blocks = 0;    // fused four-qubit blocks with a trailing permutation
+    uint32_t herm_blocks = 0;    // four-qubit blocks run as Hermitian sweeps
+    uint32_t relabels = 0;       // relabelling sweeps
+    uint32_t grouping = 0;       // 0 least growth, 1 frontier, 2 frontier scored by time
+    uint32_t tile_hist[9] = {};  // fused blocks per tile width m (4..8)
+    double predicted_us = 0.0;
+    double greedy_us = 0.0;
+};
+
 // One step of a fused plan as the executor runs it (host-side checks).
 struct PlanStepView {
     enum Kind { FUSED4, FUSED5, HERM, RELABEL, OTHER } kind;
@@ -48,6 +68,13 @@ struct FusedPlan {
     virtual int execute(double*& re, double*& im, double*& re2, double*& im2, int n,
                         bool transposed, hipStream_t stream, LaunchProfiler& prof,
                         uint32_t* launches, uint32_t* sweeps) = 0;
+    // After upload: times up to eight of the plan's four-qubit block sweeps (in
+    // place, on the scratch planes re, im) at every tile width from 5 to
+    // min(8, n) -- and 4 when every sampled block holds qubit 0 -- and gives
+    // every block the fastest width where it is admissible.  *chosen = that
+    // width (0: no block to time).  Updates info's width histogram.
+    virtual int autotune_tiles(int n, double* re, double* im, hipStream_t stream, int* chosen,
+                               PlanInfo& info) = 0;
 };
 
 // Blocks of a circuit in execution order: order[i] = (0, index into blocks)
@@ -75,8 +102,14 @@ struct BlockPlan {
 // send the block's index bits onto bits 0..3: a trailing CX then joins only if
 // its control is not a block bit (after the block's earlier trailing gates)
 // while its target is not.
+// model: choose among the least-growth grouping, the frontier grouping and the
+// frontier grouping scored by the gates' predicted unfused times, the one whose
+// blocks the network prices cheapest (each the cheaper of its fused sweep and
+// its gates unfused); without a model the one with fewer sweeps.  *grouping
+// (may be null) = the choice: 0, 1 or 2 in that order.
 int plan_blocks(const std::vector<Gate>& circuit, int n, int max_k, BlockPlan& out, bool& fusable,
-                bool allow_perm = false, bool herm = false);
+                bool allow_perm = false, bool herm = false, CostModel* model = nullptr,
+                uint32_t* grouping = nullptr);
 
 // Relative extra cost of a sweep with a trailing permutation on n qubits:
 // about 2.5 us of index-map set-up against the sweep's own time (32 * 4^n bytes
@@ -98,24 +131,7 @@ inline bool permuting_pays(int n, double sweeps, double permuting, double sweeps
     return n >= kPermuteSureQubits || sweeps + permute_overhead(n) * permuting < sweeps_inplace;
 }
 
-class CostModel;  // mlp.hpp
 
-// What the planner chose.  predicted_us is the cost model's estimate of the
-// run's launches (0 without a model); greedy_us its estimate for every block
-// fused at the default tile width.
-struct PlanInfo {
-    uint32_t fused_blocks = 0;   // four-qubit blocks run fused
-    uint32_t plain_blocks = 0;   // four-qubit blocks run as primitive sweeps on both indices
-    uint32_t fused5_blocks = 0;  // five-qubit blocks run fused (a split one counts as its
-                                 // four-qubit sub-blocks instead)
-    uint32_t channels = 0;       // noise channels (one superoperator sweep each)
-    uint32_t perm_blocks = 0;    // fused four-qubit blocks with a trailing permutation
-    uint32_t herm_blocks = 0;    // four-qubit blocks run as Hermitian sweeps
-    uint32_t relabels = 0;       // relabelling sweeps
-    uint32_t tile_hist[9] = {};  // fused blocks per tile width m (4..8)
-    double predicted_us = 0.0;
-    double greedy_us = 0.0;
-};
 
 // Plan of a circuit for the fused executor.  With a cost model, each block is
 // priced as a fused launch at every tile width and as its primitive sweeps

@@ -9,7 +9,7 @@
 
 namespace dmx {
 
-constexpr int kMlpFeatures = 13;
+constexpr int kMlpFeatures = 14;
 constexpr int kMlpHidden = 64;
 constexpr int kMlpParams = kMlpHidden * kMlpFeatures + kMlpHidden + kMlpHidden * kMlpHidden +
                            kMlpHidden + kMlpHidden + 1;
@@ -30,16 +30,18 @@ enum CostKind : int {
     CK_CX,         // k_sweep controlled swap, one side
     CK_QUAD,       // k_quad, one side
     CK_FUSED5,     // k_fused5, one sweep for both sides
+    CK_HERM,       // k_fused4_herm, one sweep for both sides (upper tiles, half writes)
 };
 
 // Feature vector of one launch (values scaled to about [0, 1]).  A sweep
 // launch on the slow index (slow = 1) acts on flattened index bit q + n.
-//  0 n/16 | 1..6 kind one-hot | 7 tile bits m/8 (fused kinds) |
-//  8 log2(workgroups launched)/16 |
-//  9 fused: block qubits below 4, /4; sweep: narrow-path flag (bit < 7) |
-//  10 fused: highest block qubit/16; sweep: target bit/32 |
-//  11 fused: lowest block qubit/16; sweep: low bit of a 4x4, else target, /32 |
-//  12 fused: contiguous low super-tile column bits/8; cx: control bit+1 /32
+//  0 n/16 | 1..7 kind one-hot | 8 tile bits m/8 (fused kinds) |
+//  9 log2(workgroups launched)/16 |
+//  10 fused: block qubits below 4, /4; sweep: narrow-path flag (bit < 7) |
+//  11 fused: highest block qubit/16; sweep: target bit/32 |
+//  12 fused: lowest block qubit/16; sweep: low bit of a 4x4, else target, /32 |
+//  13 fused: contiguous low super-tile column bits/8; cx: control bit+1 /32
+// Hermitian sweeps: m = 8, qmask = the block's qubits in the circuit's order.
 // fused: qmask = the block's four (five) qubits; sweeps: target (4x4: high
 // qubit), control (cx: control qubit; 4x4: low qubit; else -1).
 void cost_features(int n, int kind, int m, uint32_t qmask, int target, int control, int slow,

@@ -19,8 +19,8 @@ LIB_PATH = os.path.join(HERE, "libdmsim_amd.so")
 DMX_OK, DMX_EINVAL, DMX_ESTATE, DMX_EHIP, DMX_ENOMEM = 0, -1, -2, -3, -4
 NOALLOC_MAX_ENTRIES = 65536  # dmx_get_entries_noalloc limit (include/dmsim_amd.h)
 METHOD_PLAIN, METHOD_PLAIN_LAUNCH, METHOD_FUSED = 0, 1, 2
-COST_FUSED, COST_DENSE, COST_DIAG, COST_CX, COST_QUAD, COST_FUSED5 = 0, 1, 2, 3, 4, 5
-COST_FEATURES, COST_PARAMS = 13, 5121
+COST_FUSED, COST_DENSE, COST_DIAG, COST_CX, COST_QUAD, COST_FUSED5, COST_HERM = 0, 1, 2, 3, 4, 5, 6
+COST_FEATURES, COST_PARAMS = 14, 5185
 
 # every symbol include/dmsim_amd.h declares
 EXPORTS = [
@@ -29,7 +29,8 @@ EXPORTS = [
     "dmx_clear_circuit", "dmx_reset", "dmx_reset_dm", "dmx_measure", "dmx_get_dm",
     "dmx_get_entries", "dmx_get_entries_noalloc",
     "dmx_get_diag", "dmx_expect_z", "dmx_info", "dmx_dump", "dmx_set_method",
-    "dmx_set_fusion_max_k", "dmx_set_fusion_permute", "dmx_set_fusion_hermitian", "dmx_last_run_stats", "dmx_last_run_fallbacks", "dmx_set_report",
+    "dmx_set_fusion_max_k", "dmx_set_fusion_permute", "dmx_set_fusion_hermitian",
+    "dmx_set_fusion_autotune", "dmx_tuned_tile_bits", "dmx_last_run_stats", "dmx_last_run_fallbacks", "dmx_set_report",
     "dmx_set_profiling", "dmx_profile_get",
     "dmx_qasm_parse", "dmx_append_qasm",
     "dmx_cost_create", "dmx_cost_destroy", "dmx_cost_features", "dmx_cost_train",
@@ -82,6 +83,8 @@ def load():
     lib.dmx_set_fusion_max_k.argtypes = [vp, C.c_uint32]
     lib.dmx_set_fusion_permute.argtypes = [vp, C.c_uint32]
     lib.dmx_set_fusion_hermitian.argtypes = [vp, C.c_uint32]
+    lib.dmx_set_fusion_autotune.argtypes = [vp, C.c_uint32]
+    lib.dmx_tuned_tile_bits.argtypes = [vp, P(C.c_uint32)]
     lib.dmx_last_run_stats.argtypes = [vp, P(C.c_double), P(C.c_uint32), P(C.c_uint32)]
     lib.dmx_last_run_fallbacks.argtypes = [vp, P(C.c_uint32)]
     lib.dmx_set_report.argtypes = [vp, C.c_int]
@@ -104,7 +107,8 @@ def load():
     lib.dmx_cost_set_params.argtypes = [vp, f32p, C.c_float, C.c_float]
     lib.dmx_set_cost_model.argtypes = [vp, vp]
     lib.dmx_set_tile_bits.argtypes = [vp, C.c_uint32]
-    lib.dmx_plan_sweeps.argtypes = [vp, P(C.c_uint32), P(C.c_uint32), P(C.c_uint32)]
+    lib.dmx_plan_sweeps.argtypes = [vp, P(C.c_uint32), P(C.c_uint32), P(C.c_uint32),
+                                    P(C.c_uint32)]
     lib.dmx_plan_info.argtypes = [vp, P(C.c_uint32), P(C.c_uint32), P(C.c_uint32),
                                   P(C.c_double), P(C.c_double), P(C.c_uint32)]
     lib.dmx_time_launch.argtypes = [vp, C.c_int32, C.c_int32, C.c_uint32, C.c_int32, C.c_int32,
@@ -292,6 +296,16 @@ class Sim:
         (default), 2 from 8 qubits (needs permuting blocks on)."""
         _check(load().dmx_set_fusion_hermitian(self._h, int(mode)))
 
+    def set_fusion_autotune(self, on: bool):
+        """Greedy plans (no cost model): measure the tile width at upload
+        (default on)."""
+        _check(load().dmx_set_fusion_autotune(self._h, 1 if on else 0))
+
+    def tuned_tile_bits(self):
+        m = C.c_uint32()
+        _check(load().dmx_tuned_tile_bits(self._h, C.byref(m)))
+        return m.value
+
     def set_profiling(self, on: bool):
         _check(load().dmx_set_profiling(self._h, 1 if on else 0))
 
@@ -351,9 +365,11 @@ class Sim:
                 "tiles": {m: tiles[m - 4] for m in range(4, 9)}, **self.plan_sweeps()}
 
     def plan_sweeps(self):
-        pb, hb, rl = C.c_uint32(), C.c_uint32(), C.c_uint32()
-        _check(load().dmx_plan_sweeps(self._h, C.byref(pb), C.byref(hb), C.byref(rl)))
-        return {"perm_blocks": pb.value, "herm_blocks": hb.value, "relabels": rl.value}
+        pb, hb, rl, gr = C.c_uint32(), C.c_uint32(), C.c_uint32(), C.c_uint32()
+        _check(load().dmx_plan_sweeps(self._h, C.byref(pb), C.byref(hb), C.byref(rl),
+                                      C.byref(gr)))
+        return {"perm_blocks": pb.value, "herm_blocks": hb.value, "relabels": rl.value,
+                "grouping": ("least_growth", "frontier", "frontier_timed")[gr.value]}
 
     def time_launch(self, kind, m=0, qmask=0, target=0, control=-1, slow=0, reps=10):
         """Average device microseconds of one launch configuration (rho is left

@@ -172,6 +172,15 @@ int dmx_set_fusion_permute(dmx_sim* sim, uint32_t on);
  * planes.  Used from the next dmx_upload. */
 int dmx_set_fusion_hermitian(dmx_sim* sim, uint32_t mode);
 
+/* Extension: the greedy plan's tile width measured (on = 1, default): without
+ * a cost model and without Hermitian sweeps, dmx_upload times up to eight of
+ * the plan's block sweeps at every tile width on scratch planes (rho is not
+ * touched) and runs every block at the fastest width where it is admissible;
+ * off = 0 keeps the default width min(8, n).  dmx_tuned_tile_bits reports the
+ * width the last upload chose (0: none timed). */
+int dmx_set_fusion_autotune(dmx_sim* sim, uint32_t on);
+int dmx_tuned_tile_bits(const dmx_sim* sim, uint32_t* m);
+
 /* Device time of the last run() in milliseconds (HIP events on the run's
  * stream), number of kernel launches it made, and number of full-matrix
  * sweeps it performed. */
@@ -234,8 +243,8 @@ int dmx_append_qasm(dmx_sim* sim, const char* source, uint32_t* n_gates);
  * kernel launch.  The fused planner prices every block as a fused sweep at
  * each tile width and as its gates' primitive sweeps, and keeps the cheapest;
  * without a model it fuses every block at the default width (greedy). ---- */
-#define DMX_COST_FEATURES 13
-#define DMX_COST_PARAMS 5121
+#define DMX_COST_FEATURES 14
+#define DMX_COST_PARAMS 5185
 
 /* Launch kinds the model prices. */
 enum dmx_cost_kind {
@@ -244,7 +253,9 @@ enum dmx_cost_kind {
     DMX_COST_DIAG = 2,  /* diagonal primitive (only the bit-set entry changes) */
     DMX_COST_CX = 3,    /* controlled swap */
     DMX_COST_QUAD = 4,  /* 4x4 primitive */
-    DMX_COST_FUSED5 = 5 /* dense 5-qubit block, both indices, one sweep */
+    DMX_COST_FUSED5 = 5, /* dense 5-qubit block, both indices, one sweep */
+    DMX_COST_HERM = 6    /* dense 4-qubit block as a Hermitian sweep (m = 8, n >= 8): the
+                            tiles on and above the diagonal, out of place, half writes */
 };
 
 typedef struct dmx_cost dmx_cost;
@@ -306,9 +317,12 @@ int dmx_plan_info(const dmx_sim* sim, uint32_t* fused_blocks, uint32_t* plain_bl
                   uint32_t* tiles);
 
 /* Extension: sweeps of the uploaded fused plan by kind -- four-qubit blocks
- * with a trailing permutation, Hermitian block sweeps, relabelling sweeps. */
+ * with a trailing permutation, Hermitian block sweeps, relabelling sweeps --
+ * and the grouping the planner chose (0 least growth, 1 frontier, 2 frontier
+ * scored by the gates' predicted unfused times; with a cost model the one it
+ * prices cheapest, else the one with fewer sweeps of 0 and 1). */
 int dmx_plan_sweeps(const dmx_sim* sim, uint32_t* perm_blocks, uint32_t* herm_blocks,
-                    uint32_t* relabels);
+                    uint32_t* relabels, uint32_t* grouping);
 
 /* Timing-data generator: average device time (microseconds) of reps
  * back-to-back launches of one kernel configuration (arguments as in

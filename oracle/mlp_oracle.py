@@ -20,7 +20,7 @@ from __future__ import annotations
 
 import numpy as np
 
-F = 13
+F = 14
 H = 64
 P = H * F + H + H * H + H + H + 1
 

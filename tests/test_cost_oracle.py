@@ -54,33 +54,40 @@ def test_feature_encoding():
     dm = pytest.importorskip("dmsim_amd")
     # fused, n = 14, block on qubits 0..3, m = 8: column bits 0..7 contiguous
     f = dm.cost_features(14, dm.COST_FUSED, m=8, qmask=0xF)
-    assert f[0] == np.float32(14 / 16) and f[1] == 1 and f[7] == 1.0
-    assert f[8] == np.float32(np.log2(4096) / 16)    # grid capped at 4096 workgroups
-    assert f[9] == 1.0 and f[10] == np.float32(3 / 16) and f[11] == 0.0 and f[12] == 1.0
+    assert f[0] == np.float32(14 / 16) and f[1] == 1 and f[8] == 1.0
+    assert f[9] == np.float32(np.log2(4096) / 16)    # grid capped at 4096 workgroups
+    assert f[10] == 1.0 and f[11] == np.float32(3 / 16) and f[12] == 0.0 and f[13] == 1.0
     # block on qubits 10..13: four low free bits join it
     f = dm.cost_features(14, dm.COST_FUSED, m=8, qmask=0x3C00)
-    assert f[9] == 0.0 and f[12] == 0.5 and f[10] == np.float32(13 / 16)
+    assert f[10] == 0.0 and f[13] == 0.5 and f[11] == np.float32(13 / 16)
     # n = 6, m = 4: 2^(2 + 2) super-tiles; column bits = the block's qubits 0, 3, 4, 5
     f = dm.cost_features(6, dm.COST_FUSED, m=4, qmask=0b111001)
-    assert f[8] == np.float32(4 / 16) and f[12] == np.float32(1 / 8)
+    assert f[9] == np.float32(4 / 16) and f[13] == np.float32(1 / 8)
     with pytest.raises(ValueError):  # m = 4 needs qubit 0 in the block
         dm.cost_features(6, dm.COST_FUSED, m=4, qmask=0b111100)
     # five-qubit block on qubits 1, 2, 5, 6, 7 of n = 9 with m = 7: column bits
     # 0, 1, 2, 5, 6, 7 plus 3; 2^(4 + 2) super-tiles
     f = dm.cost_features(9, dm.COST_FUSED5, m=7, qmask=0b11100110)
-    assert f[6] == 1 and f[1:6].sum() == 0 and f[7] == np.float32(7 / 8)
-    assert f[8] == np.float32(6 / 16) and f[9] == np.float32(2 / 4)
-    assert f[10] == np.float32(7 / 16) and f[11] == np.float32(1 / 16) and f[12] == np.float32(4 / 8)
+    assert f[6] == 1 and f[1:6].sum() == 0 and f[8] == np.float32(7 / 8)
+    assert f[9] == np.float32(6 / 16) and f[10] == np.float32(2 / 4)
+    assert f[11] == np.float32(7 / 16) and f[12] == np.float32(1 / 16) and f[13] == np.float32(4 / 8)
     with pytest.raises(ValueError):  # m = 5 needs qubit 0 in the block
         dm.cost_features(9, dm.COST_FUSED5, m=5, qmask=0b11100110)
     with pytest.raises(ValueError):  # four qubits are not a five-qubit block
         dm.cost_features(9, dm.COST_FUSED5, m=6, qmask=0xF)
     # slow-index cx: flattened bits shifted by n
     f = dm.cost_features(10, dm.COST_CX, target=2, control=5, slow=1)
-    assert f[4] == 1 and f[10] == np.float32(12 / 32) and f[12] == np.float32(16 / 32)
-    assert f[9] == 0.0
+    assert f[4] == 1 and f[11] == np.float32(12 / 32) and f[13] == np.float32(16 / 32)
+    assert f[10] == 0.0
+    # Hermitian sweep, n = 14: 16 * 64 * 65 / 2 items, grid capped at 4096
+    f = dm.cost_features(14, dm.COST_HERM, m=8, qmask=0b10010100001)
+    assert f[7] == 1 and f[1:7].sum() == 0 and f[8] == 1.0
+    assert f[9] == np.float32(12 / 16) and f[10] == np.float32(1 / 4)
+    assert f[11] == np.float32(10 / 16) and f[12] == 0.0
+    with pytest.raises(ValueError):
+        dm.cost_features(14, dm.COST_HERM, m=7, qmask=0xF)
     f = dm.cost_features(10, dm.COST_DIAG, target=3, slow=0)
-    assert f[3] == 1 and f[9] == 1.0 and f[12] == 0.0
+    assert f[3] == 1 and f[10] == 1.0 and f[13] == 0.0
     with pytest.raises(ValueError):
         dm.cost_features(10, dm.COST_FUSED, m=8, qmask=0x7)
     with pytest.raises(ValueError):
