@@ -40,8 +40,11 @@ void Batch::free_device()
     exec_ = nullptr;
     graph_ = nullptr;
     for (void* p : {(void*)pool_, (void*)pool_alt_, (void*)d_blocks_, (void*)d_entries_,
-                    (void*)d_re_off_, (void*)d_terms_, (void*)d_jobs_, (void*)d_energy_})
+                    (void*)d_re_off_, (void*)d_terms_, (void*)d_jobs_, (void*)d_energy_,
+                    (void*)d_diag_, (void*)d_diag_off_})
         if (p) (void)hipFree(p);
+    d_diag_ = nullptr;
+    d_diag_off_ = nullptr;
     pool_ = nullptr;
     pool_alt_ = nullptr;
     d_blocks_ = nullptr;
@@ -80,8 +83,8 @@ int Batch::add_circuit(uint32_t n, std::vector<Gate> gates, uint32_t* id)
         set_error("add_circuit after upload");
         return DMX_ESTATE;
     }
-    if (n < 4 || n > 14) {
-        set_error("batched circuits need 4 <= n_qubits <= 14");
+    if (n < 4 || n > 16) {
+        set_error("batched circuits need 4 <= n_qubits <= 16");
         return DMX_EINVAL;
     }
     for (const Gate& g : gates) {
@@ -104,9 +107,58 @@ int Batch::add_circuit(uint32_t n, std::vector<Gate> gates, uint32_t* id)
     }
     Circ c;
     c.n = (int)n;
-    c.gates = std::move(gates);
+    // batched blocks have at most four qubits: a C4X runs as the gates the
+    // reference composes it from (qelib1 c4x as src/dmsim_cpu_omp.hpp:2701-2714
+    // spells it, whose primitive sweeps are the C4X's own)
+    for (const Gate& g : gates) {
+        if (g.op != OP_C4X) {
+            c.gates.push_back(g);
+            continue;
+        }
+        const uint32_t a = g.qb[0], b = g.qb[1], cc = g.qb[2], d = g.qb[3], e = g.qb[4];
+        auto one = [&](int op, uint32_t q) {
+            Gate x;
+            x.op = op;
+            x.qb[0] = q;
+            c.gates.push_back(x);
+        };
+        auto cu1 = [&](double lambda, uint32_t ctl, uint32_t tgt) {
+            Gate x;
+            x.op = OP_CU1;
+            x.qb[0] = ctl;
+            x.qb[1] = tgt;
+            x.lambda = lambda;
+            c.gates.push_back(x);
+        };
+        auto c3 = [&](int op, uint32_t t) {
+            Gate x;
+            x.op = op;
+            x.qb[0] = a, x.qb[1] = b, x.qb[2] = cc, x.qb[3] = t;
+            c.gates.push_back(x);
+        };
+        const double pi = 3.14159265358979323846;
+        one(OP_H, e);
+        cu1(-pi / 2, d, e);
+        one(OP_H, e);
+        c3(OP_C3X, d);
+        one(OP_H, d);
+        cu1(pi / 4, d, e);
+        one(OP_H, d);
+        c3(OP_C3X, d);
+        c3(OP_C3SQRTX, e);
+    }
     circs_.push_back(std::move(c));
     if (id) *id = (uint32_t)(circs_.size() - 1);
+    return DMX_OK;
+}
+
+int Batch::set_pool_limit(uint64_t bytes)
+{
+    if (uploaded_) {
+        set_error("set_pool_limit after upload");
+        return DMX_ESTATE;
+    }
+    pool_limit_ = bytes;
     return DMX_OK;
 }
 
@@ -187,17 +239,59 @@ int Batch::upload()
         c.blk_off = blocks.size();
         c.n_blocks = bp.blocks.size();
         blocks.insert(blocks.end(), bp.blocks.begin(), bp.blocks.end());
-        c.re_off = off;
-        c.im_off = off + (1ull << (2 * c.n));
-        off += 2ull << (2 * c.n);
-        re_off.push_back(c.re_off);
         max_steps = std::max(max_steps, c.n_blocks);
     }
-    pool_doubles_ = off;
-    BTRY(hipMalloc(&pool_, pool_doubles_ * sizeof(double)));
     bool any_perm = false;
     for (const FusedBlock& fb : blocks) any_perm = any_perm || fb.perm_on;
+    // waves: consecutive circuits whose rho fit one pool (and its twin for
+    // permuting blocks); every wave reuses the pool from offset 0
+    uint64_t limit = pool_limit_;
+    if (limit == 0) {
+        size_t fr = 0, tot = 0;
+        BTRY(hipMemGetInfo(&fr, &tot));
+        const uint64_t reserve = 2ull << 30;
+        limit = (fr > reserve ? fr - reserve : 0) / (any_perm ? 2 : 1);
+    }
+    const uint64_t cap = limit / sizeof(double);
+    wave_first_.clear();
+    wave_count_.clear();
+    wave_doubles_.clear();
+    uint64_t diag_total = 0;
+    for (int i = 0; i < (int)circs_.size(); i++) {
+        Circ& c = circs_[i];
+        const uint64_t need = 2ull << (2 * c.n);
+        if (need > cap) {
+            set_error("a batched circuit of " + std::to_string(c.n) +
+                      " qubits does not fit the pool limit");
+            return DMX_ENOMEM;
+        }
+        if (wave_first_.empty() || off + need > cap) {
+            wave_first_.push_back(i);
+            wave_count_.push_back(0);
+            wave_doubles_.push_back(0);
+            off = 0;
+        }
+        c.wave = (int)wave_first_.size() - 1;
+        c.re_off = off;
+        c.im_off = off + (1ull << (2 * c.n));
+        off += need;
+        wave_count_.back()++;
+        wave_doubles_.back() = off;
+        re_off.push_back(c.re_off);
+        c.diag_off = diag_total;
+        diag_total += 1ull << c.n;
+    }
+    pool_doubles_ = *std::max_element(wave_doubles_.begin(), wave_doubles_.end());
+    BTRY(hipMalloc(&pool_, pool_doubles_ * sizeof(double)));
     if (any_perm) BTRY(hipMalloc(&pool_alt_, pool_doubles_ * sizeof(double)));
+    if (waves() > 1) {
+        std::vector<uint64_t> doff;
+        for (const Circ& c : circs_) doff.push_back(c.diag_off);
+        BTRY(hipMalloc(&d_diag_, diag_total * sizeof(double)));
+        BTRY(hipMalloc(&d_diag_off_, doff.size() * sizeof(uint64_t)));
+        BTRY(hipMemcpy(d_diag_off_, doff.data(), doff.size() * sizeof(uint64_t),
+                       hipMemcpyHostToDevice));
+    }
     // which pool holds each circuit's rho before each of its blocks (static,
     // so the captured graph replays the same pointers)
     auto planes = [&](const Circ& c, bool alt, double*& re, double*& im) {
@@ -238,11 +332,12 @@ int Batch::upload()
     BTRY(hipMalloc(&d_re_off_, re_off.size() * sizeof(uint64_t)));
     BTRY(hipMalloc(&d_energy_, circs_.size() * sizeof(double)));
 
-    // launches: per block step, one per qubit count
+    // launches: per wave and block step, one per (qubit count, tile width)
     std::vector<BatchEntry> entries;
-    // grouped by (qubit count, tile width): one launch per group and step
+    for (int w = 0; w < (int)waves(); w++) {
     std::map<std::pair<int, int>, std::vector<int>> groups;
-    for (int i = 0; i < (int)circs_.size(); i++) groups[{circs_[i].n, 0}].push_back(i);
+    for (int i = wave_first_[w]; i < wave_first_[w] + wave_count_[w]; i++)
+        groups[{circs_[i].n, 0}].push_back(i);
     for (size_t s = 0; s < max_steps; s++) {
         std::map<std::pair<int, int>, Launch> step;
         for (const auto& kv : groups)
@@ -256,6 +351,7 @@ int Batch::upload()
             }
         for (auto& kv : step) {
             kv.second.entry_off = entries.size();
+            kv.second.wave = w;
             for (const auto& g : groups)
                 for (int i : g.second) {
                     const Circ& c = circs_[i];
@@ -266,6 +362,7 @@ int Batch::upload()
                 }
             launches_.push_back(kv.second);
         }
+    }
     }
     if (!entries.empty()) BTRY(hipMalloc(&d_entries_, entries.size() * sizeof(BatchEntry)));
 
@@ -388,10 +485,17 @@ int Batch::autotune(std::vector<FusedBlock>& blocks)
 
 int Batch::enqueue()
 {
-    BTRY(launch_pool_reset(pool_, pool_doubles_, d_re_off_, (int)circs_.size(), stream_));
-    for (const Launch& l : launches_)
-        BTRY(launch_fused4_batch(d_entries_ + l.entry_off, l.count, l.n, l.m, 4096, stream_));
-    BTRY(launch_pauli_energy(d_jobs_, d_terms_, (int)circs_.size(), d_energy_, stream_));
+    for (int w = 0; w < (int)waves(); w++) {
+        const int first = wave_first_[w], count = wave_count_[w];
+        BTRY(launch_pool_reset(pool_, wave_doubles_[w], d_re_off_ + first, count, stream_));
+        for (const Launch& l : launches_)
+            if (l.wave == w)
+                BTRY(launch_fused4_batch(d_entries_ + l.entry_off, l.count, l.n, l.m, 4096,
+                                         stream_));
+        BTRY(launch_pauli_energy(d_jobs_ + first, d_terms_, count, d_energy_ + first, stream_));
+        if (waves() > 1)  // the next wave reuses this pool space: keep the diagonals
+            BTRY(launch_batch_diag(d_jobs_ + first, d_diag_off_ + first, count, d_diag_, stream_));
+    }
     return DMX_OK;
 }
 
@@ -436,10 +540,33 @@ int Batch::get_dm(uint32_t id, double* re, double* im)
         return !ran_ ? DMX_ESTATE : DMX_EINVAL;
     }
     const Circ& c = circs_[id];
+    if (c.wave != (int)waves() - 1) {
+        set_error("rho of a circuit of an earlier wave is not kept (its pool space is reused); "
+                  "its diagonal is: dmx_batch_get_diag");
+        return DMX_ESTATE;
+    }
     const size_t bytes = (1ull << (2 * c.n)) * sizeof(double);
     const double* base = c.final_alt ? pool_alt_ : pool_;
     if (re) BTRY(hipMemcpy(re, base + c.re_off, bytes, hipMemcpyDeviceToHost));
     if (im) BTRY(hipMemcpy(im, base + c.im_off, bytes, hipMemcpyDeviceToHost));
+    return DMX_OK;
+}
+
+int Batch::get_diag(uint32_t id, double* out)
+{
+    if (!ran_ || id >= circs_.size() || !out) {
+        set_error(!ran_ ? "batch get_diag before run" : "bad circuit id or null output");
+        return !ran_ ? DMX_ESTATE : DMX_EINVAL;
+    }
+    const Circ& c = circs_[id];
+    const uint64_t dim = 1ull << c.n;
+    if (waves() > 1) {
+        BTRY(hipMemcpy(out, d_diag_ + c.diag_off, dim * sizeof(double), hipMemcpyDeviceToHost));
+    } else {
+        const double* base = (c.final_alt ? pool_alt_ : pool_) + c.re_off;
+        BTRY(hipMemcpy2D(out, sizeof(double), base, (dim + 1) * sizeof(double), sizeof(double),
+                         dim, hipMemcpyDeviceToHost));
+    }
     return DMX_OK;
 }
 

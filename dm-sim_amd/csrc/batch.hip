@@ -66,6 +66,25 @@ __global__ void __launch_bounds__(256) k_pauli_energy(const EnergyJob* __restric
     if (threadIdx.x == 0) out[blockIdx.x] = red[0];
 }
 
+__global__ void k_batch_diag(const EnergyJob* __restrict__ jobs,
+                             const uint64_t* __restrict__ diag_off, double* __restrict__ out)
+{
+    const EnergyJob j = jobs[blockIdx.y];
+    const uint64_t dim = 1ull << j.n;
+    double* o = out + diag_off[blockIdx.y];
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < dim;
+         i += (uint64_t)gridDim.x * blockDim.x)
+        o[i] = j.re[i * dim + i];
+}
+
+hipError_t launch_batch_diag(const EnergyJob* d_jobs, const uint64_t* d_diag_off, int count,
+                             double* d_out, hipStream_t stream)
+{
+    if (count < 1 || count > 65535) return hipErrorInvalidValue;
+    k_batch_diag<<<dim3(64, (unsigned)count), 256, 0, stream>>>(d_jobs, d_diag_off, d_out);
+    return hipGetLastError();
+}
+
 hipError_t launch_pool_reset(double* pool, uint64_t n_doubles, const uint64_t* d_re_off,
                              int count, hipStream_t stream)
 {

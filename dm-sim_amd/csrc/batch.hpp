@@ -35,6 +35,9 @@ hipError_t launch_pool_reset(double* pool, uint64_t n_doubles, const uint64_t* d
                              int count, hipStream_t stream);
 hipError_t launch_pauli_energy(const EnergyJob* d_jobs, const PauliTerm* d_terms, int count,
                                double* d_out, hipStream_t stream);
+// out[diag_off[c] + i] = Re rho_c[i][i] for the `count` circuits of jobs.
+hipError_t launch_batch_diag(const EnergyJob* d_jobs, const uint64_t* d_diag_off, int count,
+                             double* d_out, hipStream_t stream);
 
 class Batch {
 public:
@@ -49,7 +52,16 @@ public:
     int upload();
     int run();
     int energies(double* out) const;
+    // rho of a circuit: only for circuits of the last wave when the batch
+    // runs in more than one (the earlier ones' pool space is reused)
     int get_dm(uint32_t id, double* re, double* im);
+    // Re rho_ii of any circuit (kept for every wave)
+    int get_diag(uint32_t id, double* out);
+    // Device bytes one pool may take (0: free memory at upload less 2 GiB,
+    // halved when permuting blocks need a second pool).  Circuits are run in
+    // waves that fit, one after another inside a run.
+    int set_pool_limit(uint64_t bytes);
+    uint32_t waves() const { return (uint32_t)wave_first_.size(); }
     void set_graph(bool on) { use_graph_ = on; }
     // Tile-width autotuning at upload (default on): per qubit count, the
     // super-tile width whose batched block launch times fastest on this GPU;
@@ -59,7 +71,10 @@ public:
     int tile_bits(int n) const;
 
     double last_ms() const { return last_ms_; }
-    uint32_t launches() const { return (uint32_t)launches_.size() + 3; }
+    uint32_t launches() const
+    {
+        return (uint32_t)launches_.size() + (uint32_t)wave_first_.size() * (waves() > 1 ? 4 : 3);
+    }
     uint32_t circuits() const { return (uint32_t)circs_.size(); }
     uint64_t gates() const;
     // Algorithmic bytes of one run's block sweeps: read + write of both planes
@@ -75,12 +90,15 @@ private:
         size_t blk_off = 0;
         size_t n_blocks = 0;
         bool final_alt = false;  // rho ends in the second pool (odd number of permuting blocks)
+        int wave = 0;
+        uint64_t diag_off = 0;  // its diagonal in the kept diagonals (several waves)
     };
     struct Launch {
         int n;
         int m;
         int count;
         size_t entry_off;
+        int wave;
     };
     int enqueue();  // the whole run on stream_ (captured or eager)
     int autotune(std::vector<FusedBlock>& blocks);
@@ -93,6 +111,11 @@ private:
     bool autotune_ = true;
     bool ran_ = false;
     std::map<int, int> tile_bits_;  // qubit count -> super-tile width
+    uint64_t pool_limit_ = 0;
+    std::vector<int> wave_first_, wave_count_;  // circuits of each wave (contiguous ids)
+    std::vector<uint64_t> wave_doubles_;        // pool doubles each wave uses
+    double* d_diag_ = nullptr;                  // kept diagonals (several waves)
+    uint64_t* d_diag_off_ = nullptr;
 
     double* pool_ = nullptr;
     double* pool_alt_ = nullptr;  // second pool, for circuits with permuting blocks

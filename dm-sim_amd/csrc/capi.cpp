@@ -738,6 +738,25 @@ int dmx_batch_stats(const dmx_batch* b, double* ms, uint32_t* launches, uint32_t
     return DMX_OK;
 }
 
+int dmx_batch_get_diag(dmx_batch* b, uint32_t id, double* diag)
+{
+    if (!b) return null_handle();
+    return guarded([&] { return b->b.get_diag(id, diag); });
+}
+
+int dmx_batch_set_pool_limit(dmx_batch* b, uint64_t bytes)
+{
+    if (!b) return null_handle();
+    return guarded([&] { return b->b.set_pool_limit(bytes); });
+}
+
+int dmx_batch_waves(const dmx_batch* b, uint32_t* waves)
+{
+    if (!b) return null_handle();
+    if (waves) *waves = b->b.waves();
+    return DMX_OK;
+}
+
 int dmx_batch_sweep_bytes(const dmx_batch* b, double* bytes)
 {
     if (!b) return null_handle();

@@ -1412,11 +1412,12 @@ int build_fused_plan(const std::vector<Gate>& circuit, int n, int max_k, CostMod
         }
     }
     if (herm) to_hermitian(*plan, n, pi);
-    if (model && tile_bits <= 0 && !plan->blocks5.empty()) {
+    if (model && tile_bits <= 0 && max_k > 4 && !plan->blocks5.empty()) {
         // Whole-plan check: five-qubit blocks pay off only when they replace
         // enough four-qubit sweeps, which no single block's split can show.
         // The requested width stays unless the narrower plan is predicted
-        // faster by the margin.
+        // faster by the margin.  (At max_k = 4 the only five-qubit blocks are
+        // five-qubit gates, which every plan keeps: nothing to compare.)
         FusedPlan* alt = nullptr;
         PlanInfo ai;
         rc = build_fused_plan(circuit, n, 4, model, 0, allow_perm, &alt, &ai, herm);

@@ -39,7 +39,8 @@ EXPORTS = [
     "dmx_batch_create", "dmx_batch_destroy", "dmx_batch_add_circuit", "dmx_batch_add_pauli",
     "dmx_batch_upload", "dmx_batch_set_graph", "dmx_batch_set_autotune", "dmx_batch_tile_bits",
     "dmx_batch_run", "dmx_batch_energies", "dmx_batch_get_dm", "dmx_batch_stats",
-    "dmx_batch_sweep_bytes",
+    "dmx_batch_sweep_bytes", "dmx_batch_get_diag", "dmx_batch_set_pool_limit",
+    "dmx_batch_waves",
     "dmx_last_error",
 ]
 
@@ -126,6 +127,9 @@ def load():
     lib.dmx_batch_energies.argtypes = [vp, P(C.c_double)]
     lib.dmx_batch_get_dm.argtypes = [vp, C.c_uint32, P(C.c_double), P(C.c_double)]
     lib.dmx_batch_sweep_bytes.argtypes = [vp, P(C.c_double)]
+    lib.dmx_batch_get_diag.argtypes = [vp, C.c_uint32, P(C.c_double)]
+    lib.dmx_batch_set_pool_limit.argtypes = [vp, C.c_uint64]
+    lib.dmx_batch_waves.argtypes = [vp, P(C.c_uint32)]
     lib.dmx_batch_stats.argtypes = [vp, P(C.c_double), P(C.c_uint32), P(C.c_uint32),
                                     P(C.c_uint64)]
     lib.dmx_last_error.restype = C.c_char_p
@@ -552,6 +556,21 @@ class Batch:
         im = np.empty(d * d)
         _check(load().dmx_batch_get_dm(self._h, cid, _ptr(re, C.c_double), _ptr(im, C.c_double)))
         return re, im
+
+    def get_diag(self, cid):
+        out = np.empty(1 << self.sizes[cid])
+        _check(load().dmx_batch_get_diag(self._h, cid, _ptr(out, C.c_double)))
+        return out
+
+    def set_pool_limit(self, nbytes: int):
+        """Bytes one pool of circuit states may take (0: automatic); circuits
+        that do not fit together run in waves."""
+        _check(load().dmx_batch_set_pool_limit(self._h, int(nbytes)))
+
+    def waves(self):
+        w = C.c_uint32()
+        _check(load().dmx_batch_waves(self._h, C.byref(w)))
+        return w.value
 
     def stats(self):
         ms, la, nc, ng = C.c_double(), C.c_uint32(), C.c_uint32(), C.c_uint64()

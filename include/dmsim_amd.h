@@ -376,6 +376,22 @@ int dmx_batch_get_dm(dmx_batch* b, uint32_t id, double* re, double* im);
 int dmx_batch_stats(const dmx_batch* b, double* ms, uint32_t* launches, uint32_t* circuits,
                     uint64_t* gates);
 
+/* Extension: Re rho_ii of circuit id after a run, diag[2^n] (kept for every
+ * circuit, also when the batch runs in several waves). */
+int dmx_batch_get_diag(dmx_batch* b, uint32_t id, double* diag);
+
+/* Extension: device bytes one pool of circuit states may take (0, the
+ * default: the free device memory at upload less 2 GiB, halved when permuting
+ * blocks need a second pool).  Circuits that do not fit together run in
+ * waves -- consecutive circuits whose states fit, one wave after another
+ * inside every run, each reusing the pool -- and dmx_batch_get_dm then serves
+ * only the last wave's circuits (DMX_ESTATE for the others; their diagonals
+ * and energies are kept).  Call before dmx_batch_upload; dmx_batch_waves
+ * reports the number of waves after it.  Circuits take 4..16 qubits; a C4X
+ * runs as the gates qelib1 composes it from (blocks of at most four qubits). */
+int dmx_batch_set_pool_limit(dmx_batch* b, uint64_t bytes);
+int dmx_batch_waves(const dmx_batch* b, uint32_t* waves);
+
 /* Extension: algorithmic bytes of one batched run's block sweeps (read and
  * write of both planes, 32 * 4^n bytes, per block of every circuit; set at
  * dmx_batch_upload). */

@@ -152,3 +152,58 @@ def test_batch_rejects_bad_input():
         dmsim_amd._check(dmsim_amd.load().dmx_batch_add_pauli(b._h, cid, b"XYZ", 1.0))
     with pytest.raises(dmsim_amd.DmxError):
         b.run()  # before upload
+
+
+def test_batch_waves_match_one_wave(oracle):
+    """A pool limit that splits the batch into several waves (each reusing the
+    pool inside one run) gives the same energies and diagonals, bit for bit,
+    as one wave; rho stays available for the last wave's circuits only."""
+    import dmsim_amd
+    items = small_items(seed=41, count=12)
+
+    def build(limit):
+        b = dmsim_amd.Batch()
+        b.set_pool_limit(limit)
+        for circ, strings, weights in items:
+            b.add_circuit(circ, strings, weights)
+        b.upload()
+        b.run()
+        return b
+
+    one = build(0)
+    many = build(1 << 20)  # 1 MiB: a few 8-qubit states per wave
+    assert one.waves() == 1 and many.waves() >= 3
+    assert np.array_equal(one.energies(), many.energies())
+    last = None
+    for cid, (circ, _, _) in enumerate(items):
+        assert np.array_equal(one.get_diag(cid), many.get_diag(cid))
+        psi = oracle.sv(circ.n_qubits, circ.arrays())
+        assert np.max(np.abs(one.get_diag(cid) - np.abs(psi) ** 2)) <= 1e-10
+        last = cid
+    re1, im1 = one.get_dm(last)
+    re2, im2 = many.get_dm(last)
+    assert np.array_equal(re1, re2) and np.array_equal(im1, im2)
+    with pytest.raises(dmsim_amd.DmxError):
+        many.get_dm(0)
+    for _ in range(2):  # graph replays
+        many.run()
+        assert np.array_equal(one.energies(), many.energies())
+    one.close()
+    many.close()
+
+
+def test_batch_runs_c4x_circuits(oracle):
+    """Circuits with C4X (a five-qubit gate) run in a batch as the gates the
+    reference composes it from: diagonals against the state-vector oracle."""
+    import dmsim_amd
+    b = dmsim_amd.Batch()
+    tables = [W.structured(n, seed) for n, seed in ((9, 1), (10, 2), (11, 3))]
+    assert all(W.OP["C4X"] in t.ops for t in tables)
+    for t in tables:
+        b.add_circuit(t)
+    b.upload()
+    b.run()
+    for cid, t in enumerate(tables):
+        psi = oracle.sv(t.n_qubits, t.arrays())
+        assert np.max(np.abs(b.get_diag(cid) - np.abs(psi) ** 2)) <= 1e-10
+    b.close()

@@ -223,6 +223,37 @@ def test_learned_plan_with_five_qubit_blocks(dm, costdata, n):
     assert t_learned <= 1.10 * t_greedy
 
 
+@pytest.mark.parametrize("herm", [0, 2])
+def test_learned_plan_five_qubit_gate_at_max_k4(dm, costdata, herm):
+    """A five-qubit gate (C4X) forces a five-qubit block even when blocks are
+    capped at four qubits: the learned plan at max_k = 4 keeps it (there is
+    no narrower plan to compare with) and matches the plain method."""
+    import workloads as W
+    model = costdata.default_model()
+    n = 9
+    table = W.random_mixed(n, 120, seed=77)
+    table.add("C4X", [0, 3, 5, 7, 8])
+    table.extend(W.random_mixed(n, 60, seed=78))
+    print(f"cost: learned plan with C4X at max_k=4, n={n}, herm={herm}", flush=True)
+    ref = dm.Sim(n)
+    ref.set_method(dm.METHOD_PLAIN_LAUNCH)
+    ref.append_table(table)
+    ref.upload()
+    ref.run()
+    s = dm.Sim(n)
+    s.set_method(dm.METHOD_FUSED)
+    s.set_fusion_max_k(4)
+    s.set_fusion_hermitian(herm)
+    s.set_cost_model(model)
+    s.append_table(table)
+    s.upload()
+    assert s.plan_info()["fused5_blocks"] == 1
+    s.run()
+    re0, im0 = ref.get_dm()
+    re, im = s.get_dm()
+    assert max(np.abs(re - re0).max(), np.abs(im - im0).max()) <= 1e-10
+
+
 def _qasm(name):
     here = os.path.dirname(os.path.abspath(__file__))
     with open(os.path.join(here, "golden", "qasm", name)) as f:
