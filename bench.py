@@ -15,6 +15,7 @@ launch on the run's stream during the timed steps.  `cpu_baseline` times the
 oracle's CPU restatement of the reference path on a bounded sample.
 
   python bench.py                      # default: fused (greedy and learned plans), plain
+                                       # at n = 14, then the batched, n = 16 and n = 15 lines
   python bench.py --calibrate-copy     # one 2 x 2 GiB device copy (counter scaling)
 """
 import argparse
@@ -44,6 +45,155 @@ W_OP_C2 = 39  # the 4x4 matrix extension op (workloads.OP["C2"])
 
 def log(msg):
     print(msg, file=sys.stderr, flush=True)
+
+
+def roofline_of(prof, n, counters_n=14):
+    """Roofline of the kernel class with the most device time in a profile
+    (sum of per-launch HIP-event times and algorithmic bytes): its bytes per
+    launch over its mean launch time against the HBM peak.  traffic: the
+    counter-measured HBM/algorithmic ratio of that kernel times its bytes per
+    launch, where the ratio was measured at this n (else null)."""
+    kname, kv = max(prof.items(), key=lambda kv: kv[1]["ms"])
+    per_launch = kv["bytes"] / kv["calls"]
+    mean_s = kv["ms"] / kv["calls"] * 1e-3
+    achieved = per_launch / mean_s / 1e9
+    ratio = TRAFFIC_RATIO.get(kname) if n == counters_n else None
+    return {
+        "bound": "hbm",
+        "achieved": achieved,
+        "peak": HBM_PEAK_GBS,
+        "unit": "GB/s",
+        "frac": achieved / HBM_PEAK_GBS,
+        "traffic": ratio[0] * per_launch if ratio else None,
+        "traffic_source": (f"measured ratio from {ratio[1]} times this run's algorithmic bytes"
+                           if ratio else None),
+        "kernel": kname,
+        "bytes_per_launch": per_launch,
+        "mean_launch_us": mean_s * 1e6,
+        "launches": kv["calls"],
+    }
+
+
+def plan_summary(info):
+    keys = ("fused_blocks", "fused5_blocks", "plain_blocks", "herm_blocks", "relabels",
+            "perm_blocks", "grouping", "predicted_us")
+    return {k: info[k] for k in keys if k in info}
+
+
+def fusion_n16(model, n=16, n_gates=4000, seed=16):
+    """The learned-fusion configuration: n = 16 (rho 64 GiB), 4000 gates, 70 %
+    U3 and 30 % CX/RZZ/CU1 on uniform pairs.  Blocks of up to five qubits,
+    planned by the shipped cost model and by the greedy heuristic (no model:
+    grouping by sweep count, tile width measured at upload); one timed run of
+    each from |0><0| (a run takes seconds)."""
+    import dmsim_amd
+    import workloads as W
+
+    table = W.fusion_mix(n=n, n_gates=n_gates, seed=seed)
+    out = {"workload": f"fusion_mix n={n}: {len(table)} gates (70% U3, 30% CX/RZZ/CU1), "
+                       f"seed {seed}, blocks of up to 5 qubits",
+           "gates": len(table)}
+    for label in ("learned", "greedy"):
+        s = dmsim_amd.Sim(n)
+        s.set_method(METHODS["fused"])
+        s.set_fusion_max_k(5)
+        if label == "learned":
+            s.set_cost_model(model)
+        s.append_table(table)
+        s.upload()
+        wall, dev_ms, stats, prof = timed_runs(s, 1, 0)
+        out[label] = {
+            "gate_updates_per_s": len(table) / wall,
+            "ms_per_run": wall * 1e3,
+            "device_ms": dev_ms[0],
+            "launches": stats["launches"],
+            "plan": plan_summary(s.plan_info()),
+            "roofline": roofline_of(prof, n),
+        }
+        log(f"[bench] fusion n={n} {label}: {wall * 1e3:.0f} ms, "
+            f"{len(table) / wall:.0f} gate-updates/s, plan {out[label]['plan']}")
+        s.close()
+    out["learned_vs_greedy"] = (out["learned"]["gate_updates_per_s"]
+                                / out["greedy"]["gate_updates_per_s"])
+    out["roofline"] = out["learned"]["roofline"]
+    return out
+
+
+def structured_n15(model, n=15, seeds=16):
+    """The combined structured configuration: n = 15 (rho 16 GiB), per seed a
+    random basis state, QFT, a block of CCX/C3X/C4X/CSWAP, inverse QFT.  All
+    16 seeds through the learned-fused method (blocks of up to five qubits,
+    one engine, each circuit uploaded, then its reset and run timed), all 16
+    as one batch (in waves that fit HBM: 16 states of 17.2 GB, doubled for
+    out-of-place blocks, do not fit at once), and seed 0 through the plain
+    method (one launch per primitive)."""
+    import dmsim_amd
+    import workloads as W
+
+    tables = [W.structured(n, seed) for seed in range(seeds)]
+    gates = sum(len(t) for t in tables)
+    out = {"workload": f"structured n={n} x{seeds} seeds (QFT, CCX/C3X/C4X/CSWAP block, "
+                       f"inverse QFT)", "gates": gates}
+    s = dmsim_amd.Sim(n)
+    s.set_method(METHODS["fused"])
+    s.set_fusion_max_k(5)
+    s.set_cost_model(model)
+    s.set_profiling(True)
+    wall = dev = 0.0
+    prof = {}
+    for t in tables:
+        s.clear_circuit()
+        s.append_table(t)
+        s.upload()
+        t0 = time.perf_counter()
+        s.reset_dm()
+        s.run()
+        wall += time.perf_counter() - t0
+        dev += s.last_run_stats()["ms"]
+        for k, v in s.profile().items():
+            acc = prof.setdefault(k, {"calls": 0, "ms": 0.0, "bytes": 0.0})
+            for f in acc:
+                acc[f] += v[f]
+    s.close()
+    out["learned_fused"] = {"gate_updates_per_s": gates / wall, "ms_total": wall * 1e3,
+                            "device_ms_total": dev, "roofline": roofline_of(prof, n)}
+    log(f"[bench] structured n={n} x{seeds} learned-fused: {wall * 1e3:.0f} ms, "
+        f"{gates / wall:.0f} gate-updates/s")
+    b = dmsim_amd.Batch()
+    for t in tables:
+        b.add_circuit(t)
+    b.upload()
+    t0 = time.perf_counter()
+    b.run()
+    bwall = time.perf_counter() - t0
+    st = b.stats()
+    achieved = st["sweep_bytes"] / (st["ms"] * 1e-3) / 1e9
+    out["batched"] = {
+        "gate_updates_per_s": gates / bwall, "ms_total": bwall * 1e3, "device_ms": st["ms"],
+        "waves": b.waves(), "launches": st["launches"],
+        # the batch's block sweeps (32 * 4^n bytes each) over the run's device
+        # time, which also holds the resets and the diagonals' copies
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "bytes_per_run": st["sweep_bytes"], "device_ms_per_run": st["ms"]},
+    }
+    log(f"[bench] structured n={n} x{seeds} batched: {b.waves()} waves, {bwall * 1e3:.0f} ms, "
+        f"{gates / bwall:.0f} gate-updates/s")
+    b.close()
+    p = dmsim_amd.Sim(n)
+    p.set_method(METHODS["plain_launch"])
+    p.append_table(tables[0])
+    p.upload()
+    pwall, pdev, _, pprof = timed_runs(p, 1, 0)
+    p.close()
+    out["plain_seed0"] = {"gate_updates_per_s": len(tables[0]) / pwall, "ms": pwall * 1e3,
+                          "device_ms": pdev[0], "roofline": roofline_of(pprof, n)}
+    log(f"[bench] structured n={n} seed 0 plain: {pwall * 1e3:.0f} ms, "
+        f"{len(tables[0]) / pwall:.1f} gate-updates/s")
+    plain = out["plain_seed0"]["gate_updates_per_s"]
+    out["learned_fused_vs_plain"] = out["learned_fused"]["gate_updates_per_s"] / plain
+    out["batched_vs_plain"] = out["batched"]["gate_updates_per_s"] / plain
+    return out
 
 
 def timed_runs(sim, steps, warmup):
@@ -243,6 +393,8 @@ def main():
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-published", action="store_true",
                     help="skip the VQE-UCCSD comparison with the reference's published time")
+    ap.add_argument("--no-large", action="store_true",
+                    help="skip the n = 16 learned-vs-greedy and n = 15 structured x16 lines")
     ap.add_argument("--calibrate-copy", action="store_true")
     ap.add_argument("--gates", default="",
                     help="LO:HI -- run only gates LO..HI-1 of the workload (profiling runs)")
@@ -297,7 +449,13 @@ def main():
             "gates_per_launch": gates / max(stats["launches"], 1),
             # 64 * 4^n bytes per gate-update (two full sweeps in the plain method)
             "effective_GBs": 64.0 * num * gates / (dev_avg * 1e-3) / 1e9,
+            # the bytes the launches must move (a CX or diagonal sweep touches
+            # half of rho; a Hermitian sweep reads the upper tiles), per step,
+            # over the step's device time
+            "algorithmic_GBs": (sum(v["bytes"] for v in prof.values()) / args.steps
+                                / (dev_avg * 1e-3) / 1e9),
         }
+        results[mname]["algorithmic_frac"] = results[mname]["algorithmic_GBs"] / HBM_PEAK_GBS
         profiles[mname] = prof
         log(f"[bench] {mname}: {ms_step:.1f} ms/step, device {dev_avg:.1f} ms, "
             f"{results[mname]['gate_updates_per_s']:.1f} gate-updates/s")
@@ -310,11 +468,6 @@ def main():
 
     best = max(results, key=lambda k: results[k]["gate_updates_per_s"])
     r = results[best]
-    prof = profiles[best]
-    kname, kv = max(prof.items(), key=lambda kv: kv[1]["ms"])
-    per_launch_bytes = kv["bytes"] / kv["calls"]
-    mean_s = kv["ms"] / kv["calls"] * 1e-3
-    achieved = per_launch_bytes / mean_s / 1e9
     line = {
         "metric": METRIC,
         "value": r["gate_updates_per_s"],
@@ -339,27 +492,24 @@ def main():
                                                            for k in ("plain", "plain_launch")
                                                            if k in results)
                              if ("plain" in results or "plain_launch" in results) else None),
-        "roofline": {
-            "bound": "hbm",
-            "achieved": achieved,
-            "peak": HBM_PEAK_GBS,
-            "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS,
-            "traffic": (TRAFFIC_RATIO[kname][0] * per_launch_bytes if kname in TRAFFIC_RATIO
-                        else None),
-            "traffic_source": (f"measured ratio from {TRAFFIC_RATIO[kname][1]} times this run's "
-                               f"algorithmic bytes" if kname in TRAFFIC_RATIO else None),
-            "kernel": kname,
-            "bytes_per_launch": per_launch_bytes,
-            "mean_launch_us": mean_s * 1e6,
-            "launches": kv["calls"],
-        },
+        "roofline": roofline_of(profiles[best], n),
     }
     if not args.no_batch:
         log("[bench] batched configuration ...")
         line["batched"] = batched(args.steps, args.warmup)
         log(f"[bench] batched: {line['batched']['batched_gate_updates_per_s']:.0f} "
             f"gate-updates/s, {line['batched']['batched_vs_plain']:.1f}x one-by-one plain")
+    if not args.no_large:
+        import costdata
+        model = costdata.default_model()
+        if model is None:
+            log("[bench] no shipped cost model: n = 16 and n = 15 lines skipped")
+        else:
+            log("[bench] learned fusion n=16 (learned vs greedy, blocks of up to 5 qubits) ...")
+            line["fusion_n16"] = fusion_n16(model)
+            log("[bench] structured n=15 x16 (learned-fused, batched, plain seed 0) ...")
+            line["structured_n15"] = structured_n15(model)
+            model.close()
     if not args.no_published:
         log("[bench] published comparison (vqe_uccsd_n8) ...")
         line["published_comparison"] = vqe_published()
