@@ -40,6 +40,19 @@ __device__ __forceinline__ d4 mfma(double a, double b, d4 c)
     return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
 }
 
+typedef double dv2 __attribute__((ext_vector_type(2)));
+
+// Nontemporal 16-byte load and store (rho is streamed once per sweep).
+__device__ __forceinline__ dv2 ld_stream(const double* p)
+{
+    return __builtin_nontemporal_load(reinterpret_cast<const dv2*>(p));
+}
+
+__device__ __forceinline__ void st_stream(double* p, dv2 v)
+{
+    __builtin_nontemporal_store(v, reinterpret_cast<dv2*>(p));
+}
+
 // Scatter the low bits of v into the positions of `mask` (ascending).
 __device__ __forceinline__ uint64_t deposit(uint64_t v, uint64_t mask)
 {
@@ -338,36 +351,54 @@ __global__ void __launch_bounds__(256, 2) k_fused4_herm(const double* __restrict
     HermThread th;
     th.init(g, tid);
     const int lc = th.lc;
+    // Software pipeline: the next item's loads are issued before this item's
+    // transform and stores and land in registers while they run.  rho is
+    // streamed once per sweep: loads and stores are nontemporal.
+    dv2 vr[8], vi[8];
     uint64_t w = 0;
-    for (uint64_t k = 0; herm_slot(g, blockIdx.x, gridDim.x, k, w); k++) {
-        const HermItem it = herm_item(g, w);
-        {
-            double2 vr[8], vi[8];
-            int a[8];
+    bool have = herm_slot(g, blockIdx.x, gridDim.x, 0, w);
+    HermItem it;
+    if (have) {
+        it = herm_item(g, w);
 #pragma unroll
-            for (int rep = 0; rep < 8; rep++) {
-                const uint64_t off = herm_load(g, th, it, rep, a[rep]);
-                vr[rep] = *reinterpret_cast<const double2*>(re + off);
-                vi[rep] = *reinterpret_cast<const double2*>(im + off);
-            }
+        for (int rep = 0; rep < 8; rep++) {
+            int a;
+            const uint64_t off = herm_load(g, th, it, rep, a);
+            vr[rep] = ld_stream(re + off);
+            vi[rep] = ld_stream(im + off);
+        }
+    }
+    for (uint64_t k = 0; have; k++) {
 #pragma unroll
-            for (int rep = 0; rep < 8; rep++) {
-                s_re[a[rep] * kHermPitch + lc] = vr[rep].x;
-                s_re[a[rep] * kHermPitch + lc + 1] = vr[rep].y;
-                s_im[a[rep] * kHermPitch + lc] = vi[rep].x;
-                s_im[a[rep] * kHermPitch + lc + 1] = vi[rep].y;
-            }
+        for (int rep = 0; rep < 8; rep++) {
+            const int a = th.top + 2 * rep;
+            s_re[herm_lds(a, lc)] = vr[rep].x;
+            s_re[herm_lds(a, lc + 1)] = vr[rep].y;
+            s_im[herm_lds(a, lc)] = vi[rep].x;
+            s_im[herm_lds(a, lc + 1)] = vi[rep].y;
         }
         __syncthreads();
+        const HermItem cur = it;
+        have = herm_slot(g, blockIdx.x, gridDim.x, k + 1, w);
+        if (have) {
+            it = herm_item(g, w);
+#pragma unroll
+            for (int rep = 0; rep < 8; rep++) {
+                int a;
+                const uint64_t off = herm_load(g, th, it, rep, a);
+                vr[rep] = ld_stream(re + off);
+                vi[rep] = ld_stream(im + off);
+            }
+        }
 
         for (int tb = wave; tb < 16; tb += 4) {
-            if (!herm_tile_on(it, tb)) continue;
+            if (!herm_tile_on(cur, tb)) continue;
             const uint32_t base = (uint32_t)deposit((uint64_t)tb, g.pos_f);
             double tr[4], ti[4], ts[4];
 #pragma unroll
             for (int c = 0; c < 4; c++) {
-                tr[c] = s_re[j * kHermPitch + base + lcq_a[c]];
-                ti[c] = s_im[j * kHermPitch + base + lcq_a[c]];
+                tr[c] = s_re[herm_lds(j, (int)(base + lcq_a[c]))];
+                ti[c] = s_im[herm_lds(j, (int)(base + lcq_a[c]))];
                 ts[c] = tr[c] + ti[c];
             }
             d4 p1 = {0, 0, 0, 0}, p2 = {0, 0, 0, 0}, p3 = {0, 0, 0, 0};
@@ -392,8 +423,8 @@ __global__ void __launch_bounds__(256, 2) k_fused4_herm(const double* __restrict
 #pragma unroll
             for (int r = 0; r < 4; r++) {
                 const int a = 4 * r + kk;
-                s_re[a * kHermPitch + base + lcq_d] = yr[r];
-                s_im[a * kHermPitch + base + lcq_d] = yi[r];
+                s_re[herm_lds(a, (int)(base + lcq_d))] = yr[r];
+                s_im[herm_lds(a, (int)(base + lcq_d))] = yi[r];
             }
         }
         __syncthreads();
@@ -402,22 +433,18 @@ __global__ void __launch_bounds__(256, 2) k_fused4_herm(const double* __restrict
         for (int rep = 0; rep < 8; rep++) {
             uint64_t off;
             int a, c0, c1;
-            if (herm_direct(g, th, it, rep, off, a, c0, c1)) {
-                *reinterpret_cast<double2*>(re_out + off) =
-                    make_double2(s_re[a * kHermPitch + c0], s_re[a * kHermPitch + c1]);
-                *reinterpret_cast<double2*>(im_out + off) =
-                    make_double2(s_im[a * kHermPitch + c0], s_im[a * kHermPitch + c1]);
+            if (herm_direct(g, th, cur, rep, off, a, c0, c1)) {
+                st_stream(re_out + off, dv2{s_re[herm_lds(a, c0)], s_re[herm_lds(a, c1)]});
+                st_stream(im_out + off, dv2{s_im[herm_lds(a, c0)], s_im[herm_lds(a, c1)]});
             }
         }
 #pragma unroll
         for (int rep = 0; rep < 8; rep++) {
             uint64_t off;
             int a0, a1, col;
-            if (herm_mirror(g, th, it, rep, off, a0, a1, col)) {
-                *reinterpret_cast<double2*>(re_out + off) =
-                    make_double2(s_re[a0 * kHermPitch + col], s_re[a1 * kHermPitch + col]);
-                *reinterpret_cast<double2*>(im_out + off) =
-                    make_double2(-s_im[a0 * kHermPitch + col], -s_im[a1 * kHermPitch + col]);
+            if (herm_mirror(g, th, cur, rep, off, a0, a1, col)) {
+                st_stream(re_out + off, dv2{s_re[herm_lds(a0, col)], s_re[herm_lds(a1, col)]});
+                st_stream(im_out + off, dv2{-s_im[herm_lds(a0, col)], -s_im[herm_lds(a1, col)]});
             }
         }
         __syncthreads();
@@ -450,8 +477,8 @@ __global__ void __launch_bounds__(256, 2) k_relabel(const double* __restrict__ r
         for (int rep = 0; rep < 8; rep++) {
             int a;
             const uint64_t off = relabel_load(g, th, it, tid, rep, a);
-            const double2 vr = *reinterpret_cast<const double2*>(re + off);
-            const double2 vi = *reinterpret_cast<const double2*>(im + off);
+            const dv2 vr = ld_stream(re + off);
+            const dv2 vi = ld_stream(im + off);
             s_re[a * kHermPitch + lc] = vr.x;
             s_re[a * kHermPitch + lc + 1] = vr.y;
             s_im[a * kHermPitch + lc] = vi.x;
@@ -462,10 +489,8 @@ __global__ void __launch_bounds__(256, 2) k_relabel(const double* __restrict__ r
         for (int rep = 0; rep < 8; rep++) {
             int a;
             const uint64_t off = relabel_store(g, th, it, tid, rep, a);
-            *reinterpret_cast<double2*>(re_out + off) =
-                make_double2(s_re[a * kHermPitch + th.s0], s_re[a * kHermPitch + th.s1]);
-            *reinterpret_cast<double2*>(im_out + off) =
-                make_double2(s_im[a * kHermPitch + th.s0], s_im[a * kHermPitch + th.s1]);
+            st_stream(re_out + off, dv2{s_re[a * kHermPitch + th.s0], s_re[a * kHermPitch + th.s1]});
+            st_stream(im_out + off, dv2{s_im[a * kHermPitch + th.s0], s_im[a * kHermPitch + th.s1]});
         }
         __syncthreads();
     }

@@ -39,6 +39,18 @@ __host__ __device__ inline uint64_t deposit_bits(uint64_t v, uint64_t mask)
 // LDS row pitch (doubles) of the 16 x 256 staging planes.
 constexpr int kHermPitch = 257;
 
+// LDS place (doubles from the plane's start) of super-tile row a, column x in
+// the Hermitian sweep: the column's high nibble is XORed into its low nibble
+// (a bijection on 0..255).  Where the block's bits sit at super-tile column
+// positions 4..7 (the block above the four low free bits), a tile's 16
+// columns are 16 doubles apart and the transform's result writes -- one row,
+// 16 columns per instruction -- would hit 8 of the 32 double-wide LDS banks;
+// the XOR spreads them over all of them.
+__host__ __device__ inline int herm_lds(int a, int x)
+{
+    return a * kHermPitch + (x ^ ((x >> 4) & 15));
+}
+
 struct HermGeom {
     int n;
     uint64_t dim;

@@ -148,14 +148,14 @@ static bool run_herm(const dmx::FusedBlock& blk, int n, const std::vector<cd>& i
                 const int lc = th[tid].lc;
                 const uint64_t off = dmx::herm_load(g, th[tid], it, rep, a);
                 if (off + 1 >= num) return *why = "load out of range", false;
-                lds[a * dmx::kHermPitch + lc] = in[off];
-                lds[a * dmx::kHermPitch + lc + 1] = in[off + 1];
+                lds[dmx::herm_lds(a, lc)] = in[off];
+                lds[dmx::herm_lds(a, lc + 1)] = in[off + 1];
             }
         for (int tb = 0; tb < 16; tb++) {
             if (!dmx::herm_tile_on(it, tb)) continue;
             cd T[16][16], X[16][16];
             for (int a = 0; a < 16; a++)
-                for (int j = 0; j < 16; j++) T[a][j] = lds[a * dmx::kHermPitch + g.lds_col(tb, j)];
+                for (int j = 0; j < 16; j++) T[a][j] = lds[dmx::herm_lds(a, g.lds_col(tb, j))];
             for (int a = 0; a < 16; a++)  // X = T U^T
                 for (int j = 0; j < 16; j++) {
                     cd s = 0.0;
@@ -166,7 +166,7 @@ static bool run_herm(const dmx::FusedBlock& blk, int n, const std::vector<cd>& i
                 for (int j = 0; j < 16; j++) {
                     cd s = 0.0;
                     for (int k = 0; k < 16; k++) s += cd(blk.ur[a * 16 + k], -blk.ui[a * 16 + k]) * X[k][j];
-                    lds[a * dmx::kHermPitch + g.lds_col(tb, j)] = s;
+                    lds[dmx::herm_lds(a, g.lds_col(tb, j))] = s;
                 }
         }
         for (int tid = 0; tid < 256; tid++)
@@ -175,14 +175,14 @@ static bool run_herm(const dmx::FusedBlock& blk, int n, const std::vector<cd>& i
                 int a, c0, c1, a0, a1, col;
                 if (dmx::herm_direct(g, th[tid], it, rep, off, a, c0, c1)) {
                     if (off + 1 >= num || (off & 1)) return *why = "direct store out of range", false;
-                    out[off] = lds[a * dmx::kHermPitch + c0];
-                    out[off + 1] = lds[a * dmx::kHermPitch + c1];
+                    out[off] = lds[dmx::herm_lds(a, c0)];
+                    out[off + 1] = lds[dmx::herm_lds(a, c1)];
                     hits[off]++, hits[off + 1]++;
                 }
                 if (dmx::herm_mirror(g, th[tid], it, rep, off, a0, a1, col)) {
                     if (off + 1 >= num || (off & 1)) return *why = "mirror store out of range", false;
-                    out[off] = std::conj(lds[a0 * dmx::kHermPitch + col]);
-                    out[off + 1] = std::conj(lds[a1 * dmx::kHermPitch + col]);
+                    out[off] = std::conj(lds[dmx::herm_lds(a0, col)]);
+                    out[off + 1] = std::conj(lds[dmx::herm_lds(a1, col)]);
                     hits[off]++, hits[off + 1]++;
                 }
             }
