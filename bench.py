@@ -124,9 +124,11 @@ def structured_n15(model, n=15, seeds=16):
     random basis state, QFT, a block of CCX/C3X/C4X/CSWAP, inverse QFT.  All
     16 seeds through the learned-fused method (blocks of up to five qubits,
     one engine, each circuit uploaded, then its reset and run timed), all 16
-    as one batch (in waves that fit HBM: 16 states of 17.2 GB, doubled for
-    out-of-place blocks, do not fit at once), and seed 0 through the plain
-    method (one launch per primitive)."""
+    as one batch with the same cost model (in waves that fit HBM: 16 states
+    of 17.2 GB, doubled for out-of-place sweeps, do not fit at once; each
+    15-qubit circuit runs on its own in its wave through its learned
+    Hermitian plan), and seed 0 through the plain method (one launch per
+    primitive)."""
     import dmsim_amd
     import workloads as W
 
@@ -160,6 +162,7 @@ def structured_n15(model, n=15, seeds=16):
     log(f"[bench] structured n={n} x{seeds} learned-fused: {wall * 1e3:.0f} ms, "
         f"{gates / wall:.0f} gate-updates/s")
     b = dmsim_amd.Batch()
+    b.set_cost_model(model)
     for t in tables:
         b.add_circuit(t)
     b.upload()
@@ -171,8 +174,8 @@ def structured_n15(model, n=15, seeds=16):
     out["batched"] = {
         "gate_updates_per_s": gates / bwall, "ms_total": bwall * 1e3, "device_ms": st["ms"],
         "waves": b.waves(), "launches": st["launches"],
-        # the batch's block sweeps (32 * 4^n bytes each) over the run's device
-        # time, which also holds the resets and the diagonals' copies
+        # the bytes of the batch's sweeps over the run's device time, which
+        # also holds the resets and the diagonals' copies
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
                      "bytes_per_run": st["sweep_bytes"], "device_ms_per_run": st["ms"]},

@@ -1,11 +1,13 @@
 // batch.cpp -- batched executor (host side).
 //
 // Each circuit is planned into <= 4-qubit blocks by the fused planner.  Run
-// order: reset every rho in the pool; for block step s = 0, 1, ..., one
-// launch per (qubit count, tile width) covering every circuit of that width
-// that still has a block s (grid.y = circuit); then one workgroup per circuit
-// sums its Pauli terms.  The sequence is captured once into a hipGraph and
-// replayed by each run.
+// order, per wave: reset every rho of the wave; for block step s = 0, 1, ...,
+// one launch per (qubit count, tile width) covering every circuit of that
+// width that still has a block s (grid.y = circuit); then each circuit of
+// kSoloQubits or more, whose rho alone fills the GPU, through its own fused
+// plan (the single-circuit engine's: Hermitian block sweeps); then one
+// workgroup per circuit sums its Pauli terms.  The sequence is captured once
+// into a hipGraph and replayed by each run.
 #include "batch.hpp"
 
 #include <algorithm>
@@ -66,14 +68,15 @@ int Batch::init()
 uint64_t Batch::gates() const
 {
     uint64_t g = 0;
-    for (const Circ& c : circs_) g += c.gates.size();
+    for (const Circ& c : circs_) g += c.orig.size();
     return g;
 }
 
 double Batch::sweep_bytes() const
 {
     double b = 0;
-    for (const Circ& c : circs_) b += (double)c.n_blocks * 32.0 * (double)(1ull << (2 * c.n));
+    for (const Circ& c : circs_)
+        b += c.plan ? c.plan->bytes(c.n) : (double)c.n_blocks * 32.0 * (double)(1ull << (2 * c.n));
     return b;
 }
 
@@ -107,6 +110,7 @@ int Batch::add_circuit(uint32_t n, std::vector<Gate> gates, uint32_t* id)
     }
     Circ c;
     c.n = (int)n;
+    c.orig = gates;
     // batched blocks have at most four qubits: a C4X runs as the gates the
     // reference composes it from (qelib1 c4x as src/dmsim_cpu_omp.hpp:2701-2714
     // spells it, whose primitive sweeps are the C4X's own)
@@ -209,7 +213,37 @@ int Batch::upload()
     std::vector<uint64_t> re_off;
     uint64_t off = 0;
     size_t max_steps = 0;
+    bool any_perm = false;
     for (Circ& c : circs_) {
+        c.blk_off = blocks.size();
+        c.n_blocks = 0;
+        if (c.n >= kSoloQubits) {
+            // planned as the single-circuit engine plans it alone (fused
+            // method, blocks of up to four qubits, Hermitian sweeps)
+            FusedPlan* p = nullptr;
+            PlanInfo pi;
+            int rc = build_fused_plan(c.orig, c.n, 4, model_, 0, true, &p, &pi, true);
+            if (rc) return rc;
+            if (!p) {
+                set_error("batched circuits must be fusable (no SRN)");
+                return DMX_EINVAL;
+            }
+            c.plan.reset(p);
+            rc = p->upload(stream_);
+            if (rc) return rc;
+            c.solo = true;
+            any_perm = any_perm || p->needs_alt();
+            // where rho ends: each out-of-place step moves it to the other pool
+            std::vector<PlanStepView> steps;
+            p->describe(steps);
+            bool alt = false;
+            for (const PlanStepView& v : steps)
+                if (v.kind == PlanStepView::HERM || v.kind == PlanStepView::RELABEL ||
+                    (v.kind == PlanStepView::FUSED4 && v.block->perm_on))
+                    alt = !alt;
+            c.final_alt = alt;
+            continue;
+        }
         // the single-circuit engine's choice (same planner, same rule), so a
         // batched rho is bit-identical to the circuit's own fused run
         BlockPlan bp;
@@ -236,12 +270,10 @@ int Batch::upload()
                 set_error("batched circuits take gates on at most 4 qubits");
                 return DMX_EINVAL;
             }
-        c.blk_off = blocks.size();
         c.n_blocks = bp.blocks.size();
         blocks.insert(blocks.end(), bp.blocks.begin(), bp.blocks.end());
         max_steps = std::max(max_steps, c.n_blocks);
     }
-    bool any_perm = false;
     for (const FusedBlock& fb : blocks) any_perm = any_perm || fb.perm_on;
     // waves: consecutive circuits whose rho fit one pool (and its twin for
     // permuting blocks); every wave reuses the pool from offset 0
@@ -302,6 +334,7 @@ int Batch::upload()
     std::vector<std::vector<char>> alt_before(circs_.size());
     for (size_t i = 0; i < circs_.size(); i++) {
         Circ& c = circs_[i];
+        if (c.solo) continue;  // final_alt set with its plan
         bool alt = false;
         for (size_t b = 0; b < c.n_blocks; b++) {
             alt_before[i].push_back(alt ? 1 : 0);
@@ -492,6 +525,22 @@ int Batch::enqueue()
             if (l.wave == w)
                 BTRY(launch_fused4_batch(d_entries_ + l.entry_off, l.count, l.n, l.m, 4096,
                                          stream_));
+        for (int i = first; i < first + count; i++) {
+            Circ& c = circs_[i];
+            if (!c.solo) continue;
+            double* re = pool_ + c.re_off;
+            double* im = pool_ + c.im_off;
+            double* re2 = pool_alt_ ? pool_alt_ + c.re_off : nullptr;
+            double* im2 = pool_alt_ ? pool_alt_ + c.im_off : nullptr;
+            uint32_t la = 0, sw = 0;
+            const int rc = c.plan->execute(re, im, re2, im2, c.n, false, stream_, quiet_, &la, &sw);
+            if (rc) return rc;
+            if (re != (c.final_alt ? pool_alt_ : pool_) + c.re_off) {
+                set_error("internal: a batched circuit's rho ends in the other pool");
+                return DMX_EINVAL;
+            }
+            c.solo_launches = la;
+        }
         BTRY(launch_pauli_energy(d_jobs_ + first, d_terms_, count, d_energy_ + first, stream_));
         if (waves() > 1)  // the next wave reuses this pool space: keep the diagonals
             BTRY(launch_batch_diag(d_jobs_ + first, d_diag_off_ + first, count, d_diag_, stream_));

@@ -5,12 +5,14 @@
 
 #include <cstdint>
 #include <map>
+#include <memory>
 #include <string>
 #include <vector>
 
 #include "fuse.hpp"
 #include "fused.hpp"
 #include "lower.hpp"
+#include "prof.hpp"
 
 namespace dmx {
 
@@ -69,22 +71,38 @@ public:
     int set_autotune(bool on);
     // Super-tile width used for circuits of n qubits (0 if none in the batch).
     int tile_bits(int n) const;
+    // Planner network for the circuits that run on their own (used at upload;
+    // null: the greedy planner).
+    void set_cost_model(CostModel* m) { model_ = m; }
+    // From this many qubits a circuit's rho alone fills the GPU: it runs on
+    // its own inside its wave, planned as the single-circuit engine plans it
+    // (Hermitian block sweeps, five-qubit gates kept whole), instead of in
+    // the batched block launches.
+    static constexpr int kSoloQubits = kHermAutoQubits;
 
     double last_ms() const { return last_ms_; }
     uint32_t launches() const
     {
-        return (uint32_t)launches_.size() + (uint32_t)wave_first_.size() * (waves() > 1 ? 4 : 3);
+        uint32_t solo = 0;
+        for (const Circ& c : circs_) solo += c.solo_launches;
+        return (uint32_t)launches_.size() + solo +
+               (uint32_t)wave_first_.size() * (waves() > 1 ? 4 : 3);
     }
     uint32_t circuits() const { return (uint32_t)circs_.size(); }
     uint64_t gates() const;
     // Algorithmic bytes of one run's block sweeps: read + write of both planes
-    // (32 * 4^n bytes) per block of every circuit.
+    // (32 * 4^n bytes) per batched block, and what the steps of the circuits
+    // that run on their own move.
     double sweep_bytes() const;
 
 private:
     struct Circ {
         int n;
-        std::vector<Gate> gates;
+        std::vector<Gate> orig;   // as added
+        std::vector<Gate> gates;  // C4X expanded (batched blocks have at most four qubits)
+        bool solo = false;        // runs on its own (kSoloQubits and up)
+        std::unique_ptr<FusedPlan> plan;  // its plan then
+        uint32_t solo_launches = 0;
         std::vector<PauliTerm> paulis;
         uint64_t re_off = 0, im_off = 0;
         size_t blk_off = 0;
@@ -106,6 +124,8 @@ private:
 
     std::vector<Circ> circs_;
     std::vector<Launch> launches_;
+    CostModel* model_ = nullptr;
+    LaunchProfiler quiet_;  // disabled: the solo plans' executor times nothing
     bool uploaded_ = false;
     bool use_graph_ = true;
     bool autotune_ = true;

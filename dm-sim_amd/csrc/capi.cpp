@@ -23,9 +23,12 @@ struct dmx_sim {
 // A model remembers the simulations that plan with it, so destroying it
 // first detaches it from them (their later uploads, including the one a
 // method switch makes, plan greedily instead of reading freed memory).
+struct dmx_batch;
+
 struct dmx_cost {
     dmx::CostModel m;
     std::set<dmx_sim*> users;
+    std::set<dmx_batch*> batches;
 };
 
 static_assert(DMX_COST_FEATURES == dmx::kMlpFeatures, "feature count");
@@ -33,6 +36,7 @@ static_assert(DMX_COST_PARAMS == dmx::kMlpParams, "parameter count");
 
 struct dmx_batch {
     dmx::Batch b;
+    dmx_cost* model = nullptr;  // the planner model set on this batch
 };
 
 namespace {
@@ -465,11 +469,16 @@ int dmx_cost_create(uint32_t seed, dmx_cost** out)
 
 int dmx_cost_destroy(dmx_cost* model)
 {
-    if (model)
+    if (model) {
         for (dmx_sim* s : model->users) {
             s->eng.set_cost_model(nullptr);
             s->model = nullptr;
         }
+        for (dmx_batch* b : model->batches) {
+            b->b.set_cost_model(nullptr);
+            b->model = nullptr;
+        }
+    }
     delete model;
     return DMX_OK;
 }
@@ -645,7 +654,18 @@ int dmx_batch_create(dmx_batch** out)
 
 int dmx_batch_destroy(dmx_batch* b)
 {
+    if (b && b->model) b->model->batches.erase(b);
     delete b;
+    return DMX_OK;
+}
+
+int dmx_batch_set_cost_model(dmx_batch* b, dmx_cost* model)
+{
+    if (!b) return null_handle();
+    if (b->model) b->model->batches.erase(b);
+    b->model = model;
+    if (model) model->batches.insert(b);
+    b->b.set_cost_model(model ? &model->m : nullptr);
     return DMX_OK;
 }
 

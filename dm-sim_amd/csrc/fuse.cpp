@@ -172,6 +172,25 @@ struct Plan : FusedPlan {
         }
     }
 
+    double bytes(int nq) const override
+    {
+        const double sweep = 32.0 * (double)(1ull << (2 * nq));
+        double b = 0;
+        for (const Step& st : steps) {
+            if (st.type == ST_HERM) {
+                b += herm_bytes(nq, blocks[st.index].next_q != 0);
+            } else if (st.type == ST_RELABEL || st.type == ST_FUSED4 || st.type == ST_FUSED5) {
+                b += sweep;
+            } else if (st.type == ST_CHANNEL) {
+                b += prim_bytes(prims[st.index], nq);
+            } else {
+                for (int t = st.index; t < st.index + st.count; t++)
+                    b += 2.0 * prim_bytes(prims[t], nq);
+            }
+        }
+        return b;
+    }
+
     int upload(hipStream_t stream) override
     {
         int rc = put(&d_blocks, blocks, stream);

@@ -75,6 +75,9 @@ struct FusedPlan {
     // width (0: no block to time).  Updates info's width histogram.
     virtual int autotune_tiles(int n, double* re, double* im, hipStream_t stream, int* chosen,
                                PlanInfo& info) = 0;
+    // Algorithmic bytes of one execute() on n qubits (rho not transposed):
+    // the sum of what its launches record in the profiler.
+    virtual double bytes(int n) const = 0;
 };
 
 // Blocks of a circuit in execution order: order[i] = (0, index into blocks)

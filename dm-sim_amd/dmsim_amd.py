@@ -40,7 +40,7 @@ EXPORTS = [
     "dmx_batch_upload", "dmx_batch_set_graph", "dmx_batch_set_autotune", "dmx_batch_tile_bits",
     "dmx_batch_run", "dmx_batch_energies", "dmx_batch_get_dm", "dmx_batch_stats",
     "dmx_batch_sweep_bytes", "dmx_batch_get_diag", "dmx_batch_set_pool_limit",
-    "dmx_batch_waves",
+    "dmx_batch_waves", "dmx_batch_set_cost_model",
     "dmx_last_error",
 ]
 
@@ -130,6 +130,7 @@ def load():
     lib.dmx_batch_get_diag.argtypes = [vp, C.c_uint32, P(C.c_double)]
     lib.dmx_batch_set_pool_limit.argtypes = [vp, C.c_uint64]
     lib.dmx_batch_waves.argtypes = [vp, P(C.c_uint32)]
+    lib.dmx_batch_set_cost_model.argtypes = [vp, vp]
     lib.dmx_batch_stats.argtypes = [vp, P(C.c_double), P(C.c_uint32), P(C.c_uint32),
                                     P(C.c_uint64)]
     lib.dmx_last_error.restype = C.c_char_p
@@ -571,6 +572,12 @@ class Batch:
         w = C.c_uint32()
         _check(load().dmx_batch_waves(self._h, C.byref(w)))
         return w.value
+
+    def set_cost_model(self, model):
+        """Planner model for the circuits of 12 or more qubits, which run on
+        their own (None = greedy plan); used at upload."""
+        self._model = model  # keep it alive while this batch may plan with it
+        _check(load().dmx_batch_set_cost_model(self._h, model._h if model is not None else None))
 
     def stats(self):
         ms, la, nc, ng = C.c_double(), C.c_uint32(), C.c_uint32(), C.c_uint64()

@@ -387,15 +387,25 @@ int dmx_batch_get_diag(dmx_batch* b, uint32_t id, double* diag);
  * inside every run, each reusing the pool -- and dmx_batch_get_dm then serves
  * only the last wave's circuits (DMX_ESTATE for the others; their diagonals
  * and energies are kept).  Call before dmx_batch_upload; dmx_batch_waves
- * reports the number of waves after it.  Circuits take 4..16 qubits; a C4X
- * runs as the gates qelib1 composes it from (blocks of at most four qubits). */
+ * reports the number of waves after it.  Circuits take 4..16 qubits.  Below
+ * 12 qubits a C4X runs as the gates qelib1 composes it from (batched blocks
+ * have at most four qubits); a circuit of 12 or more qubits, whose state
+ * alone fills the GPU, runs on its own inside its wave, planned as dmx_upload
+ * plans it alone with the fused method at its defaults (Hermitian block
+ * sweeps, five-qubit gates kept whole), with the batch's cost model if one is
+ * set. */
 int dmx_batch_set_pool_limit(dmx_batch* b, uint64_t bytes);
 int dmx_batch_waves(const dmx_batch* b, uint32_t* waves);
 
-/* Extension: algorithmic bytes of one batched run's block sweeps (read and
- * write of both planes, 32 * 4^n bytes, per block of every circuit; set at
- * dmx_batch_upload). */
+/* Extension: algorithmic bytes of one batched run's sweeps (read and write
+ * of both planes, 32 * 4^n bytes, per batched block; for a circuit run on its
+ * own, what its plan's launches move), set at dmx_batch_upload. */
 int dmx_batch_sweep_bytes(const dmx_batch* b, double* bytes);
+
+/* Extension: the planner network for the batch's circuits of 12 or more
+ * qubits (those that run on their own; null: the greedy planner), used at
+ * dmx_batch_upload.  Destroying the model detaches it. */
+int dmx_batch_set_cost_model(dmx_batch* b, dmx_cost* model);
 
 /* Message of the last failure on this thread ("" if none). */
 const char* dmx_last_error(void);

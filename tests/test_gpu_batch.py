@@ -207,3 +207,57 @@ def test_batch_runs_c4x_circuits(oracle):
         psi = oracle.sv(t.n_qubits, t.arrays())
         assert np.max(np.abs(b.get_diag(cid) - np.abs(psi) ** 2)) <= 1e-10
     b.close()
+
+
+@pytest.mark.parametrize("learned", [False, True])
+def test_batch_large_circuits_run_on_their_own(oracle, learned):
+    """Circuits of 12 or more qubits run on their own inside their wave,
+    through the single-circuit engine's fused plan (Hermitian block sweeps, a
+    C4X kept as one five-qubit block), greedy or with the cost model: rho
+    bit-identical to the circuit's own fused run; energies and diagonals
+    against the state-vector oracle; the small circuit of the same wave still
+    runs batched."""
+    import costdata
+    import dmsim_amd
+    model = costdata.default_model() if learned else None
+    rng = np.random.default_rng(1212)
+    tables = [W.structured(12, 6), W.structured(9, 5), W.random_mixed(12, 300, seed=12),
+              W.structured(13, 8)]
+    items = []
+    for t in tables:
+        strings, weights = W.random_paulis(t.n_qubits, 6, rng)
+        items.append((t, strings, weights))
+    print(f"batch: large circuits on their own, learned={learned}", flush=True)
+    b = dmsim_amd.Batch()
+    if model is not None:
+        b.set_cost_model(model)
+    for circ, strings, weights in items:
+        b.add_circuit(circ, strings, weights)
+    b.upload()
+    b.run()
+    assert b.waves() == 1
+    en = b.energies()
+    for cid, (t, strings, weights) in enumerate(items):
+        n = t.n_qubits
+        psi = oracle.sv(n, t.arrays())
+        assert np.max(np.abs(b.get_diag(cid) - np.abs(psi) ** 2)) <= 1e-10, cid
+        assert abs(en[cid] - sv_energy(psi, strings, weights)) < 1e-9, cid
+        if n < 12:
+            continue
+        s = dmsim_amd.Sim(n)
+        s.set_method(dmsim_amd.METHOD_FUSED)
+        if model is not None:
+            s.set_cost_model(model)
+        s.append_table(t)
+        s.upload()
+        assert s.plan_info()["herm_blocks"] > 0
+        s.run()
+        re, im = s.get_dm()
+        bre, bim = b.get_dm(cid)
+        assert np.array_equal(re, bre) and np.array_equal(im, bim), cid
+        s.close()
+    b.run()  # graph replay
+    assert np.array_equal(en, b.energies())
+    b.close()
+    if model is not None:
+        model.close()

@@ -81,11 +81,12 @@ def test_structured_n15(oracle, sim15, seed):
     assert abs(total - 1.0) <= 1e-12
 
 
-def test_structured_n15_plain_and_batched(oracle, sim15):
+def test_structured_n15_plain_and_batched(oracle, sim15, model):
     """Seed 0 through the plain method (one launch per primitive) against the
     learned-fused run, and all 16 seeds as one batch (in waves of four: 16
-    states of 17.2 GB do not fit one MI355X together) against the
-    state-vector oracle: diagonals within 1e-10, shots drawn from each
+    states of 17.2 GB do not fit one MI355X together; each 15-qubit circuit
+    runs on its own in its wave through the learned Hermitian plan) against
+    the state-vector oracle: diagonals within 1e-10, shots drawn from each
     diagonal identical to those drawn from the oracle's |psi|^2."""
     import dmsim_amd
     n = 15
@@ -118,6 +119,7 @@ def test_structured_n15_plain_and_batched(oracle, sim15):
     p.close()
 
     b = dmsim_amd.Batch()
+    b.set_cost_model(model)
     tables = [W.structured(n, seed) for seed in range(16)]
     for tb in tables:
         b.add_circuit(tb)
