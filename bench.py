@@ -294,6 +294,7 @@ def batched(steps, warmup, seed=4, n_circuits=256):
         if tuned:
             out["launches_per_run"] = st["launches"]
             out["tile_bits"] = {n: b.tile_bits(n) for n in range(6, 12)}
+            out["schedule"] = b.schedule()
             # the batched run's block sweeps: 32 * 4^n bytes each, over the
             # run's device time (which also holds the reset and the energies)
             out["roofline"] = {

@@ -212,7 +212,6 @@ int Batch::upload()
     std::vector<PauliTerm> terms;
     std::vector<uint64_t> re_off;
     uint64_t off = 0;
-    size_t max_steps = 0;
     bool any_perm = false;
     for (Circ& c : circs_) {
         c.blk_off = blocks.size();
@@ -272,7 +271,6 @@ int Batch::upload()
             }
         c.n_blocks = bp.blocks.size();
         blocks.insert(blocks.end(), bp.blocks.begin(), bp.blocks.end());
-        max_steps = std::max(max_steps, c.n_blocks);
     }
     for (const FusedBlock& fb : blocks) any_perm = any_perm || fb.perm_on;
     // waves: consecutive circuits whose rho fit one pool (and its twin for
@@ -365,38 +363,61 @@ int Batch::upload()
     BTRY(hipMalloc(&d_re_off_, re_off.size() * sizeof(uint64_t)));
     BTRY(hipMalloc(&d_energy_, circs_.size() * sizeof(double)));
 
-    // launches: per wave and block step, one per (qubit count, tile width)
-    std::vector<BatchEntry> entries;
-    for (int w = 0; w < (int)waves(); w++) {
-    std::map<std::pair<int, int>, std::vector<int>> groups;
-    for (int i = wave_first_[w]; i < wave_first_[w] + wave_count_[w]; i++)
-        groups[{circs_[i].n, 0}].push_back(i);
-    for (size_t s = 0; s < max_steps; s++) {
-        std::map<std::pair<int, int>, Launch> step;
-        for (const auto& kv : groups)
-            for (int i : kv.second) {
-                const Circ& c = circs_[i];
-                if (s >= c.n_blocks) continue;
-                const int m = (int)blocks[c.blk_off + s].m;
-                auto it = step.find({c.n, m});
-                if (it == step.end()) it = step.insert({{c.n, m}, Launch{c.n, m, 0, 0}}).first;
-                it->second.count++;
-            }
-        for (auto& kv : step) {
-            kv.second.entry_off = entries.size();
-            kv.second.wave = w;
-            for (const auto& g : groups)
-                for (int i : g.second) {
-                    const Circ& c = circs_[i];
-                    if (s >= c.n_blocks || c.n != kv.first.first ||
-                        (int)blocks[c.blk_off + s].m != kv.first.second)
-                        continue;
-                    entries.push_back(entry(i, s));
+    // Launches, per wave: its batched circuits packed into groups whose rho
+    // take at most `budget` bytes (largest circuits first; 0: the whole wave
+    // is one group), and per group and block step one launch per (qubit
+    // count, tile width).  A group runs all its steps before the next starts,
+    // so with a budget below the Infinity Cache its rho stay in the cache
+    // from one step to the next.  Returns the number of groups.
+    auto build = [&](uint64_t budget, std::vector<BatchEntry>& ents, std::vector<Launch>& lst) {
+        size_t n_groups = 0;
+        for (int w = 0; w < (int)waves(); w++) {
+            std::vector<int> ids;
+            for (int i = wave_first_[w]; i < wave_first_[w] + wave_count_[w]; i++)
+                if (circs_[i].n_blocks > 0) ids.push_back(i);
+            if (budget > 0)
+                std::stable_sort(ids.begin(), ids.end(),
+                                 [&](int a, int b) { return circs_[a].n > circs_[b].n; });
+            size_t k = 0;
+            while (k < ids.size()) {
+                std::vector<int> grp;
+                uint64_t bytes = 0;
+                while (k < ids.size()) {
+                    const uint64_t need = 16ull << (2 * circs_[ids[k]].n);
+                    if (budget > 0 && !grp.empty() && bytes + need > budget) break;
+                    grp.push_back(ids[k++]);
+                    bytes += need;
                 }
-            launches_.push_back(kv.second);
+                std::sort(grp.begin(), grp.end(), [&](int a, int b) {
+                    return circs_[a].n != circs_[b].n ? circs_[a].n < circs_[b].n : a < b;
+                });
+                n_groups++;
+                size_t steps = 0;
+                for (int i : grp) steps = std::max(steps, circs_[i].n_blocks);
+                for (size_t s = 0; s < steps; s++) {
+                    std::map<std::pair<int, int>, std::vector<int>> by;
+                    for (int i : grp)
+                        if (s < circs_[i].n_blocks)
+                            by[{circs_[i].n, (int)blocks[circs_[i].blk_off + s].m}].push_back(i);
+                    for (const auto& kv : by) {
+                        lst.push_back(Launch{kv.first.first, kv.first.second,
+                                             (int)kv.second.size(), ents.size(), w});
+                        for (int i : kv.second) ents.push_back(entry(i, s));
+                    }
+                }
+            }
+        }
+        return n_groups;
+    };
+    if (!group_fixed_) {
+        group_bytes_ = 0;
+        if (autotune_ && !blocks.empty()) {
+            const int rc = autotune_groups(build);
+            if (rc) return rc;
         }
     }
-    }
+    std::vector<BatchEntry> entries;
+    groups_ = (uint32_t)build(group_bytes_, entries, launches_);
     if (!entries.empty()) BTRY(hipMalloc(&d_entries_, entries.size() * sizeof(BatchEntry)));
 
     std::vector<EnergyJob> jobs;
@@ -436,6 +457,17 @@ int Batch::upload()
     BTRY(hipGraphInstantiate(&exec_, graph_, nullptr, nullptr, 0));
     energy_.assign(circs_.size(), 0.0);
     uploaded_ = true;
+    return DMX_OK;
+}
+
+int Batch::set_group_bytes(uint64_t bytes)
+{
+    if (uploaded_) {
+        set_error("set_group_bytes after upload");
+        return DMX_ESTATE;
+    }
+    group_bytes_ = bytes;
+    group_fixed_ = true;
     return DMX_OK;
 }
 
@@ -513,6 +545,59 @@ int Batch::autotune(std::vector<FusedBlock>& blocks)
     if (rc) return rc;
     for (const Circ& c : circs_)
         for (size_t b = 0; b < c.n_blocks; b++) blocks[c.blk_off + b].m = (uint32_t)tile_bits_[c.n];
+    return DMX_OK;
+}
+
+// Times the batched block launches of one run (after a warm-up pass) under
+// each group budget -- the whole wave as one group, and 16, 32, 64 and 128 MiB
+// of rho per group (well inside the 256 MB Infinity Cache, which also holds
+// the second pool of permuting blocks) -- and keeps the fastest.  The sweeps
+// act on the zeroed pools; the schedule changes no circuit's arithmetic.
+int Batch::autotune_groups(const LaunchBuilder& build)
+{
+    const uint64_t MiB = 1ull << 20;
+    const uint64_t cands[] = {0, 16 * MiB, 32 * MiB, 64 * MiB, 128 * MiB};
+    constexpr int kCands = 5;
+    std::vector<BatchEntry> ents[kCands];
+    std::vector<Launch> lsts[kCands];
+    size_t most = 1;
+    for (int c = 0; c < kCands; c++) {
+        build(cands[c], ents[c], lsts[c]);
+        most = std::max(most, ents[c].size());
+    }
+    BTRY(hipMemsetAsync(pool_, 0, pool_doubles_ * sizeof(double), stream_));
+    if (pool_alt_) BTRY(hipMemsetAsync(pool_alt_, 0, pool_doubles_ * sizeof(double), stream_));
+    BTRY(hipStreamSynchronize(stream_));
+    BatchEntry* d_tmp = nullptr;
+    BTRY(hipMalloc(&d_tmp, most * sizeof(BatchEntry)));
+    float best = 0.f;
+    int pick = 0;
+    hipError_t e = hipSuccess;
+    for (int c = 0; c < kCands && e == hipSuccess; c++) {
+        // the previous candidate's launches have finished (event synchronised)
+        e = hipMemcpy(d_tmp, ents[c].data(), ents[c].size() * sizeof(BatchEntry),
+                      hipMemcpyHostToDevice);
+        for (int rep = 0; rep < 2 && e == hipSuccess; rep++) {  // rep 0 warms up
+            if (rep == 1) e = hipEventRecord(ev0_, stream_);
+            for (const Launch& l : lsts[c])
+                if (e == hipSuccess)
+                    e = launch_fused4_batch(d_tmp + l.entry_off, l.count, l.n, l.m, 4096, stream_);
+        }
+        if (e == hipSuccess) e = hipEventRecord(ev1_, stream_);
+        if (e == hipSuccess) e = hipEventSynchronize(ev1_);
+        float ms = 0.f;
+        if (e == hipSuccess) e = hipEventElapsedTime(&ms, ev0_, ev1_);
+        if (e == hipSuccess && (c == 0 || ms < best)) {
+            best = ms;
+            pick = c;
+        }
+    }
+    (void)hipFree(d_tmp);
+    if (e != hipSuccess) {
+        set_error(std::string("batch schedule timing: ") + hipGetErrorString(e));
+        return DMX_EHIP;
+    }
+    group_bytes_ = cands[pick];
     return DMX_OK;
 }
 

@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <functional>
 #include <map>
 #include <memory>
 #include <string>
@@ -71,6 +72,14 @@ public:
     int set_autotune(bool on);
     // Super-tile width used for circuits of n qubits (0 if none in the batch).
     int tile_bits(int n) const;
+    // Schedule chosen at upload: rho bytes per group of batched circuits that
+    // run all their block steps before the next group (0: each wave is one
+    // group), and the number of groups.  Timed among several budgets when
+    // autotuning is on; otherwise 0.
+    uint64_t group_bytes() const { return group_bytes_; }
+    uint32_t groups() const { return groups_; }
+    // Fixes the group budget (bytes, 0 = whole waves) instead of timing it.
+    int set_group_bytes(uint64_t bytes);
     // Planner network for the circuits that run on their own (used at upload;
     // null: the greedy planner).
     void set_cost_model(CostModel* m) { model_ = m; }
@@ -118,7 +127,10 @@ private:
         size_t entry_off;
         int wave;
     };
+    using LaunchBuilder =
+        std::function<size_t(uint64_t, std::vector<BatchEntry>&, std::vector<Launch>&)>;
     int enqueue();  // the whole run on stream_ (captured or eager)
+    int autotune_groups(const LaunchBuilder& build);
     int autotune(std::vector<FusedBlock>& blocks);
     void free_device();
 
@@ -132,6 +144,9 @@ private:
     bool ran_ = false;
     std::map<int, int> tile_bits_;  // qubit count -> super-tile width
     uint64_t pool_limit_ = 0;
+    uint64_t group_bytes_ = 0;
+    uint32_t groups_ = 0;
+    bool group_fixed_ = false;
     std::vector<int> wave_first_, wave_count_;  // circuits of each wave (contiguous ids)
     std::vector<uint64_t> wave_doubles_;        // pool doubles each wave uses
     double* d_diag_ = nullptr;                  // kept diagonals (several waves)

@@ -659,6 +659,20 @@ int dmx_batch_destroy(dmx_batch* b)
     return DMX_OK;
 }
 
+int dmx_batch_schedule(const dmx_batch* b, uint64_t* group_bytes, uint32_t* groups)
+{
+    if (!b) return null_handle();
+    if (group_bytes) *group_bytes = b->b.group_bytes();
+    if (groups) *groups = b->b.groups();
+    return DMX_OK;
+}
+
+int dmx_batch_set_group_bytes(dmx_batch* b, uint64_t bytes)
+{
+    if (!b) return null_handle();
+    return b->b.set_group_bytes(bytes);
+}
+
 int dmx_batch_set_cost_model(dmx_batch* b, dmx_cost* model)
 {
     if (!b) return null_handle();

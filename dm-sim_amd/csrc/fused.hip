@@ -35,6 +35,16 @@ typedef double d4 __attribute__((ext_vector_type(4)));
 
 constexpr int kPitch = 257;  // LDS row pitch in doubles (odd: spreads rows over banks)
 
+// LDS place of super-tile row a, column x (x < 2^M): the column's high nibble
+// XORed into its low nibble (a bijection on 0..2^M-1).  A block on qubits
+// above the super-tile's low free bits keeps a tile's 16 columns 16 doubles
+// apart, so the transform's result writes (one row, 16 columns) would hit 8
+// of the 32 double-wide banks; the XOR spreads them.
+__device__ __forceinline__ int lds_at(int a, int x)
+{
+    return a * kPitch + (x ^ ((x >> 4) & 15));
+}
+
 __device__ __forceinline__ d4 mfma(double a, double b, d4 c)
 {
     return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
@@ -73,7 +83,7 @@ __device__ __forceinline__ uint64_t deposit(uint64_t v, uint64_t mask)
 // PERM: the block has a trailing permutation pi; super-tiles are read from
 // (re, im) and written, entry (i, j) to (pi(i), pi(j)), into (re_out, im_out).
 // Otherwise re_out/im_out are re/im (in place).
-template <int M, bool PERM = false>
+template <int M, bool PERM = false, bool PIPE = false>
 __device__ __forceinline__ void fused_sweep(double* re, double* im, double* re_out,
                                             double* im_out, const FusedBlock* __restrict__ blk,
                                             int n, uint64_t st_first, uint64_t st_stride,
@@ -172,33 +182,42 @@ __device__ __forceinline__ void fused_sweep(double* re, double* im, double* re_o
         l_col = lin((uint32_t)coloff);
     }
 
+    // PIPE (rho streamed from HBM): software pipeline, the next super-tile's
+    // loads are issued once this one is in LDS and land in registers during
+    // the transform and stores.  Without it (rho in the Infinity Cache, few
+    // super-tiles per workgroup) each super-tile is loaded when its turn comes.
+    double2 vr[iters], vi[iters];
+    auto load = [&](uint64_t g) {
+#pragma unroll
+        for (int it = 0; it < iters; it++) {
+            vr[it] = *reinterpret_cast<const double2*>(re + g + rowoff[it]);
+            vi[it] = *reinterpret_cast<const double2*>(im + g + rowoff[it]);
+        }
+    };
+    if (PIPE && active && st_first < n_st) load(base_of(st_first));
     for (uint64_t st = st_first; st < n_st; st += st_stride) {
         const uint64_t g0 = base_of(st);
         if (active) {
-            double2 vr[iters], vi[iters];
-#pragma unroll
-            for (int it = 0; it < iters; it++) {
-                vr[it] = *reinterpret_cast<const double2*>(re + g0 + rowoff[it]);
-                vi[it] = *reinterpret_cast<const double2*>(im + g0 + rowoff[it]);
-            }
+            if (!PIPE) load(g0);
 #pragma unroll
             for (int it = 0; it < iters; it++) {
                 const int a = (2 * tid + 512 * it) >> m;
-                s_re[a * kPitch + lc] = vr[it].x;
-                s_re[a * kPitch + lc + 1] = vr[it].y;
-                s_im[a * kPitch + lc] = vi[it].x;
-                s_im[a * kPitch + lc + 1] = vi[it].y;
+                s_re[lds_at(a, lc)] = vr[it].x;
+                s_re[lds_at(a, lc + 1)] = vr[it].y;
+                s_im[lds_at(a, lc)] = vi[it].x;
+                s_im[lds_at(a, lc + 1)] = vi[it].y;
             }
         }
         __syncthreads();
+        if (PIPE && active && st + st_stride < n_st) load(base_of(st + st_stride));
 
         for (int tb = wave; tb < ntiles; tb += 4) {
             const uint32_t base = (uint32_t)deposit((uint64_t)tb, pos_f);
             double tr[4], ti[4], ts[4];
 #pragma unroll
             for (int c = 0; c < 4; c++) {
-                tr[c] = s_re[j * kPitch + base + lcq_a[c]];
-                ti[c] = s_im[j * kPitch + base + lcq_a[c]];
+                tr[c] = s_re[lds_at(j, (int)(base + lcq_a[c]))];
+                ti[c] = s_im[lds_at(j, (int)(base + lcq_a[c]))];
                 ts[c] = tr[c] + ti[c];
             }
             // X = T U^T
@@ -225,8 +244,8 @@ __device__ __forceinline__ void fused_sweep(double* re, double* im, double* re_o
 #pragma unroll
             for (int r = 0; r < 4; r++) {
                 const int a = 4 * r + kk;
-                s_re[a * kPitch + base + lcq_d] = yr[r];
-                s_im[a * kPitch + base + lcq_d] = yi[r];
+                s_re[lds_at(a, (int)(base + lcq_d))] = yr[r];
+                s_im[lds_at(a, (int)(base + lcq_d))] = yi[r];
             }
         }
         __syncthreads();
@@ -236,9 +255,9 @@ __device__ __forceinline__ void fused_sweep(double* re, double* im, double* re_o
             for (int it = 0; it < iters; it++) {
                 const int a = (2 * tid + 512 * it) >> m;
                 *reinterpret_cast<double2*>(re + g0 + rowoff[it]) =
-                    make_double2(s_re[a * kPitch + lc], s_re[a * kPitch + lc + 1]);
+                    make_double2(s_re[lds_at(a, lc)], s_re[lds_at(a, lc + 1)]);
                 *reinterpret_cast<double2*>(im + g0 + rowoff[it]) =
-                    make_double2(s_im[a * kPitch + lc], s_im[a * kPitch + lc + 1]);
+                    make_double2(s_im[lds_at(a, lc)], s_im[lds_at(a, lc + 1)]);
             }
         } else if (active) {
             const uint64_t rg = st >> cg_bits;
@@ -252,8 +271,8 @@ __device__ __forceinline__ void fused_sweep(double* re, double* im, double* re_o
             for (int it = 0; it < iters; it++) {
                 const int a = (2 * tid + 512 * it) >> m;
                 const uint64_t off = (uint64_t)(pr ^ l_row[it]) * dim + dcol;
-                const double r0 = s_re[a * kPitch + lc], r1 = s_re[a * kPitch + lc + 1];
-                const double i0 = s_im[a * kPitch + lc], i1 = s_im[a * kPitch + lc + 1];
+                const double r0 = s_re[lds_at(a, lc)], r1 = s_re[lds_at(a, lc + 1)];
+                const double i0 = s_im[lds_at(a, lc)], i1 = s_im[lds_at(a, lc + 1)];
                 *reinterpret_cast<double2*>(re_out + off) =
                     sw ? make_double2(r1, r0) : make_double2(r0, r1);
                 *reinterpret_cast<double2*>(im_out + off) =
@@ -264,16 +283,16 @@ __device__ __forceinline__ void fused_sweep(double* re, double* im, double* re_o
     }
 }
 
-template <int M>
+template <int M, bool PIPE>
 __global__ void __launch_bounds__(256, 2) k_fused4(double* __restrict__ re, double* __restrict__ im,
                                                     const FusedBlock* __restrict__ blk, int n)
 {
     __shared__ double s_re[16 * kPitch];
     __shared__ double s_im[16 * kPitch];
-    fused_sweep<M>(re, im, re, im, blk, n, blockIdx.x, gridDim.x, s_re, s_im);
+    fused_sweep<M, false, PIPE>(re, im, re, im, blk, n, blockIdx.x, gridDim.x, s_re, s_im);
 }
 
-template <int M>
+template <int M, bool PIPE>
 __global__ void __launch_bounds__(256, 2) k_fused4_perm(const double* __restrict__ re,
                                                          const double* __restrict__ im,
                                                          double* __restrict__ re_out,
@@ -282,8 +301,8 @@ __global__ void __launch_bounds__(256, 2) k_fused4_perm(const double* __restrict
 {
     __shared__ double s_re[16 * kPitch];
     __shared__ double s_im[16 * kPitch];
-    fused_sweep<M, true>(const_cast<double*>(re), const_cast<double*>(im), re_out, im_out, blk,
-                         n, blockIdx.x, gridDim.x, s_re, s_im);
+    fused_sweep<M, true, PIPE>(const_cast<double*>(re), const_cast<double*>(im), re_out, im_out,
+                               blk, n, blockIdx.x, gridDim.x, s_re, s_im);
 }
 
 // Batched form: blockIdx.y selects one circuit's (rho, block) pair, all of
@@ -500,6 +519,10 @@ namespace {
 
 uint64_t supertiles(int n, int m) { return (1ull << (n - 4)) * (1ull << (n - m)); }
 
+// From this many qubits rho (>= 268 MB) streams from HBM rather than staying
+// in the Infinity Cache: the in-order sweeps pipeline their loads.
+constexpr int kPipeQubits = 12;
+
 }  // namespace
 
 hipError_t launch_fused4(double* re, double* im, const FusedBlock* d_blk, int n, int m,
@@ -508,12 +531,18 @@ hipError_t launch_fused4(double* re, double* im, const FusedBlock* d_blk, int n,
     if (n < 4 || m < 4 || m > 8 || m > n) return hipErrorInvalidValue;
     const uint64_t n_st = supertiles(n, m);
     const unsigned blocks = (unsigned)(n_st < (uint64_t)max_blocks ? n_st : (uint64_t)max_blocks);
-    switch (m) {
-    case 4: k_fused4<4><<<blocks, 256, 0, stream>>>(re, im, d_blk, n); break;
-    case 5: k_fused4<5><<<blocks, 256, 0, stream>>>(re, im, d_blk, n); break;
-    case 6: k_fused4<6><<<blocks, 256, 0, stream>>>(re, im, d_blk, n); break;
-    case 7: k_fused4<7><<<blocks, 256, 0, stream>>>(re, im, d_blk, n); break;
-    default: k_fused4<8><<<blocks, 256, 0, stream>>>(re, im, d_blk, n); break;
+    const bool pipe = n >= kPipeQubits;
+    switch (m * 2 + (pipe ? 1 : 0)) {
+    case 8: k_fused4<4, false><<<blocks, 256, 0, stream>>>(re, im, d_blk, n); break;
+    case 9: k_fused4<4, true><<<blocks, 256, 0, stream>>>(re, im, d_blk, n); break;
+    case 10: k_fused4<5, false><<<blocks, 256, 0, stream>>>(re, im, d_blk, n); break;
+    case 11: k_fused4<5, true><<<blocks, 256, 0, stream>>>(re, im, d_blk, n); break;
+    case 12: k_fused4<6, false><<<blocks, 256, 0, stream>>>(re, im, d_blk, n); break;
+    case 13: k_fused4<6, true><<<blocks, 256, 0, stream>>>(re, im, d_blk, n); break;
+    case 14: k_fused4<7, false><<<blocks, 256, 0, stream>>>(re, im, d_blk, n); break;
+    case 15: k_fused4<7, true><<<blocks, 256, 0, stream>>>(re, im, d_blk, n); break;
+    case 16: k_fused4<8, false><<<blocks, 256, 0, stream>>>(re, im, d_blk, n); break;
+    default: k_fused4<8, true><<<blocks, 256, 0, stream>>>(re, im, d_blk, n); break;
     }
     return hipGetLastError();
 }
@@ -526,13 +555,27 @@ hipError_t launch_fused4_perm(const double* re, const double* im, double* re_out
         return hipErrorInvalidValue;
     const uint64_t n_st = supertiles(n, m);
     const unsigned blocks = (unsigned)(n_st < (uint64_t)max_blocks ? n_st : (uint64_t)max_blocks);
+    const bool pipe = n >= kPipeQubits;
+#define DMX_PERM_CASE(MM)                                                                  \
+    case MM:                                                                              \
+        if (pipe)                                                                         \
+            k_fused4_perm<MM, true><<<blocks, 256, 0, stream>>>(re, im, re_out, im_out, d_blk, n); \
+        else                                                                              \
+            k_fused4_perm<MM, false><<<blocks, 256, 0, stream>>>(re, im, re_out, im_out, d_blk, n); \
+        break;
     switch (m) {
-    case 4: k_fused4_perm<4><<<blocks, 256, 0, stream>>>(re, im, re_out, im_out, d_blk, n); break;
-    case 5: k_fused4_perm<5><<<blocks, 256, 0, stream>>>(re, im, re_out, im_out, d_blk, n); break;
-    case 6: k_fused4_perm<6><<<blocks, 256, 0, stream>>>(re, im, re_out, im_out, d_blk, n); break;
-    case 7: k_fused4_perm<7><<<blocks, 256, 0, stream>>>(re, im, re_out, im_out, d_blk, n); break;
-    default: k_fused4_perm<8><<<blocks, 256, 0, stream>>>(re, im, re_out, im_out, d_blk, n); break;
+        DMX_PERM_CASE(4)
+        DMX_PERM_CASE(5)
+        DMX_PERM_CASE(6)
+        DMX_PERM_CASE(7)
+    default:
+        if (pipe)
+            k_fused4_perm<8, true><<<blocks, 256, 0, stream>>>(re, im, re_out, im_out, d_blk, n);
+        else
+            k_fused4_perm<8, false><<<blocks, 256, 0, stream>>>(re, im, re_out, im_out, d_blk, n);
+        break;
     }
+#undef DMX_PERM_CASE
     return hipGetLastError();
 }
 

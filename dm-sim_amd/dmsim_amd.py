@@ -40,7 +40,8 @@ EXPORTS = [
     "dmx_batch_upload", "dmx_batch_set_graph", "dmx_batch_set_autotune", "dmx_batch_tile_bits",
     "dmx_batch_run", "dmx_batch_energies", "dmx_batch_get_dm", "dmx_batch_stats",
     "dmx_batch_sweep_bytes", "dmx_batch_get_diag", "dmx_batch_set_pool_limit",
-    "dmx_batch_waves", "dmx_batch_set_cost_model",
+    "dmx_batch_waves", "dmx_batch_set_cost_model", "dmx_batch_schedule",
+    "dmx_batch_set_group_bytes",
     "dmx_last_error",
 ]
 
@@ -131,6 +132,8 @@ def load():
     lib.dmx_batch_set_pool_limit.argtypes = [vp, C.c_uint64]
     lib.dmx_batch_waves.argtypes = [vp, P(C.c_uint32)]
     lib.dmx_batch_set_cost_model.argtypes = [vp, vp]
+    lib.dmx_batch_schedule.argtypes = [vp, P(C.c_uint64), P(C.c_uint32)]
+    lib.dmx_batch_set_group_bytes.argtypes = [vp, C.c_uint64]
     lib.dmx_batch_stats.argtypes = [vp, P(C.c_double), P(C.c_uint32), P(C.c_uint32),
                                     P(C.c_uint64)]
     lib.dmx_last_error.restype = C.c_char_p
@@ -572,6 +575,19 @@ class Batch:
         w = C.c_uint32()
         _check(load().dmx_batch_waves(self._h, C.byref(w)))
         return w.value
+
+    def set_group_bytes(self, nbytes: int):
+        """Fixes the schedule's group budget (0 = whole waves) instead of
+        timing it at upload."""
+        _check(load().dmx_batch_set_group_bytes(self._h, int(nbytes)))
+
+    def schedule(self):
+        """The schedule chosen at upload: rho bytes per group of batched
+        circuits run step after step together (0: whole waves) and the number
+        of groups."""
+        gb, ng = C.c_uint64(), C.c_uint32()
+        _check(load().dmx_batch_schedule(self._h, C.byref(gb), C.byref(ng)))
+        return {"group_bytes": gb.value, "groups": ng.value}
 
     def set_cost_model(self, model):
         """Planner model for the circuits of 12 or more qubits, which run on

@@ -402,6 +402,20 @@ int dmx_batch_waves(const dmx_batch* b, uint32_t* waves);
  * own, what its plan's launches move), set at dmx_batch_upload. */
 int dmx_batch_sweep_bytes(const dmx_batch* b, double* bytes);
 
+/* Extension: the schedule dmx_batch_upload chose.  Within each wave the
+ * batched circuits (below 12 qubits) are packed, largest first, into groups
+ * whose states take at most *group_bytes bytes; each group runs all its block
+ * steps before the next, so its states stay in the 256 MB Infinity Cache
+ * between steps.  With autotuning on (dmx_batch_set_autotune, the default)
+ * the budget is the fastest of the whole wave (0) and 16, 32, 64 and 128 MiB,
+ * timed at upload; off, 0.  *groups = the number of groups. */
+int dmx_batch_schedule(const dmx_batch* b, uint64_t* group_bytes, uint32_t* groups);
+
+/* Extension: fixes the group budget (bytes of state per group, 0 = each wave
+ * one group) instead of timing it; before dmx_batch_upload.  The schedule
+ * changes no circuit's arithmetic: results are bit-identical under any. */
+int dmx_batch_set_group_bytes(dmx_batch* b, uint64_t bytes);
+
 /* Extension: the planner network for the batch's circuits of 12 or more
  * qubits (those that run on their own; null: the greedy planner), used at
  * dmx_batch_upload.  Destroying the model detaches it. */

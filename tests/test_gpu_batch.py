@@ -261,3 +261,44 @@ def test_batch_large_circuits_run_on_their_own(oracle, learned):
     b.close()
     if model is not None:
         model.close()
+
+
+def test_batch_cache_groups_are_bit_identical(oracle):
+    """Circuits packed into groups that run all their block steps before the
+    next group (a 1 MiB budget: many groups) give energies and rho bit for bit
+    equal to whole-wave steps; the timed schedule (autotuning) picks one of
+    the budgets and agrees as well."""
+    import dmsim_amd
+    rng = np.random.default_rng(77)
+    items = []
+    for _ in range(14):
+        n = int(rng.integers(6, 11))
+        circ = W.hea_circuit(n, int(rng.integers(4, 12)), rng)
+        strings, weights = W.random_paulis(n, 6, rng)
+        items.append((circ, strings, weights))
+
+    def build(budget):
+        b = dmsim_amd.Batch()
+        if budget is not None:
+            b.set_group_bytes(budget)
+        for circ, strings, weights in items:
+            b.add_circuit(circ, strings, weights)
+        b.upload()
+        b.run()
+        return b
+
+    whole, grouped, timed = build(0), build(1 << 20), build(None)
+    sw, sg, st = whole.schedule(), grouped.schedule(), timed.schedule()
+    print(f"batch schedules: whole {sw}, 1 MiB {sg}, timed {st}", flush=True)
+    assert sw["groups"] == 1 and sg["groups"] > 4 and sg["group_bytes"] == 1 << 20
+    assert st["group_bytes"] in (0, 16 << 20, 32 << 20, 64 << 20, 128 << 20)
+    for other in (grouped, timed):
+        assert np.array_equal(whole.energies(), other.energies())
+        for cid in range(len(items)):
+            a, b = whole.get_dm(cid), other.get_dm(cid)
+            assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1]), cid
+    for cid, (circ, strings, weights) in enumerate(items):
+        psi = oracle.sv(circ.n_qubits, circ.arrays())
+        assert abs(whole.energies()[cid] - sv_energy(psi, strings, weights)) < 1e-9
+    for b in (whole, grouped, timed):
+        b.close()
