@@ -329,6 +329,100 @@ struct PB {
 };
 using Seq = std::vector<std::pair<int, int>>;  // (0, index into blocks) or (3, channel gate)
 
+// Index-bit order of rho along a Hermitian plan.  sigma[q] is the index bit
+// holding logical qubit q, sinv its inverse (the identity at the start and
+// after every relabelling).
+struct BitOrder {
+    std::vector<int> sigma, sinv;
+    explicit BitOrder(int n) : sigma(n), sinv(n)
+    {
+        for (int q = 0; q < n; q++) sigma[q] = sinv[q] = q;
+    }
+    bool home() const
+    {
+        for (size_t q = 0; q < sigma.size(); q++)
+            if (sigma[q] != (int)q) return false;
+        return true;
+    }
+    void go_home()
+    {
+        for (size_t q = 0; q < sigma.size(); q++) sigma[q] = sinv[q] = (int)q;
+    }
+    uint32_t phys(uint32_t logical) const
+    {
+        uint32_t o = 0;
+        for (size_t q = 0; q < sigma.size(); q++)
+            if ((logical >> q) & 1u) o |= 1u << sigma[q];
+        return o;
+    }
+};
+
+// Linear part of a block's trailing permutation (its X, CX and SWAP gates
+// after the matrix gates) by columns on logical qubits: Lp[q] = pi(2^q) ^
+// pi(0); the identity without trailing gates.  n <= 16.
+void trailing_linear(const std::vector<Gate>& circuit, const PB& pb, int n, uint32_t* Lp)
+{
+    auto pi = [&](uint32_t x) {
+        if (pb.dense < 0) return x;
+        for (size_t t = pb.dense; t < pb.gates.size(); t++) {
+            const Gate& g = circuit[pb.gates[t]];
+            const uint32_t a = g.qb[0], c = g.qb[1];
+            if (g.op == OP_X) {
+                x ^= 1u << a;
+            } else if (g.op == OP_CX) {
+                x ^= ((x >> a) & 1u) << c;
+            } else {  // SWAP
+                const uint32_t d = ((x >> a) ^ (x >> c)) & 1u;
+                x ^= (d << a) | (d << c);
+            }
+        }
+        return x;
+    };
+    const uint32_t b0 = pi(0);
+    for (int q = 0; q < 16; q++) Lp[q] = q < n ? pi(1u << q) ^ b0 : 0u;
+}
+
+// A Hermitian sweep of the four-qubit block on logical qubits Q with trailing
+// map pi (linear part Lp, by columns, on logical qubits) under bit order bo:
+// false if pi(span Q) is no set of bits (the block then runs in order);
+// otherwise *qphys = the index bits the sweep reads the block on and the order
+// it writes: the qubits spanning pi(span Q) at bits 0..3 (ascending), then the
+// images of the super-tile's four low free bits, then the rest by their
+// current bit.
+bool herm_step(int n, const BitOrder& bo, uint32_t Q, const uint32_t* Lp, uint32_t* qphys,
+               BitOrder* next)
+{
+    uint32_t img = 0;
+    for (uint32_t m = Q; m; m &= m - 1) img |= Lp[__builtin_ctz(m)];
+    if (__builtin_popcount(img) != 4) return false;
+    const uint32_t qp = bo.phys(Q);
+    *qphys = qp;
+    if (!next) return true;
+    uint32_t cm = qp;
+    for (int q = 0; q < n && __builtin_popcount(cm) < 8; q++) cm |= 1u << q;
+    std::vector<int> order;
+    std::vector<char> placed(n, 0);
+    auto place = [&](int q) {
+        if (!placed[q]) {
+            placed[q] = 1;
+            order.push_back(q);
+        }
+    };
+    for (int q = 0; q < n; q++)
+        if ((img >> q) & 1u) place(q);
+    for (int p = 0; p < n; p++)
+        if (((cm & ~qp) >> p) & 1u) {
+            const uint32_t t = Lp[bo.sinv[p]];
+            if (__builtin_popcount(t) == 1) place(__builtin_ctz(t));
+        }
+    for (int p = 0; p < n; p++) place(bo.sinv[p]);
+    for (int i = 0; i < n; i++) {
+        next->sigma[order[i]] = i;
+        next->sinv[i] = order[i];
+    }
+    return true;
+}
+
 // Grouping 1, in program order: each gate joins the open block that must take
 // the fewest new qubits (the latest such block on ties) among the blocks at or
 // after the last one touching its qubits -- the blocks in between do not touch
@@ -419,15 +513,17 @@ void commute_classes(const Gate& g, uint32_t& zm, uint32_t& xm)
 // gate, 1 per one-qubit gate (those join almost any block later).  With
 // allow_perm, a chosen four-qubit block also takes the X, CX and SWAP gates
 // that become ready after it, as the permutation of its output (they follow
-// all of its matrix gates, so the order is still a valid one).  The kLook best
-// candidates, each with the permutation gates it would take, are compared by
+// all of its matrix gates, so the order is still a valid one).  The best few
+// candidates (kLook, kLookDeep from 12 qubits), each with the permutation
+// gates it would take, are compared by
 // their score plus the best score of the block that would follow them; the
 // smaller Q wins ties.  A gate is looked for at most kWindow places past the
 // first ungrouped gate of each qubit.
 class FrontierGrouper {
 public:
     FrontierGrouper(const std::vector<Gate>& circuit, int n, int kmax, bool allow_perm, bool herm)
-        : circuit_(circuit), n_(n), kmax_(kmax), herm_(herm), lists_(n), head_(n, 0),
+        : circuit_(circuit), n_(n), kmax_(kmax), herm_(herm),
+          look_(n >= kHermAutoQubits ? kLookDeep : kLook), lists_(n), head_(n, 0),
           mask_(circuit.size(), 0), zm_(circuit.size(), 0), xm_(circuit.size(), 0),
           pos0_(circuit.size(), 0), done_(circuit.size(), 0), mark_(circuit.size(), 0),
           listed_(circuit.size(), 0), base_(n), work_(n)
@@ -463,7 +559,7 @@ public:
         // lookahead costs about kLook + 1 searches per block: skipped for very
         // long segments to bound planning time
         const bool small = seg.size() <= kLookMaxGates;
-        const int look = small ? kLook : 0;
+        const int look = small ? look_ : 0;
         const int cap = small ? kMaxExplored : kMaxExploredLong;
         size_t left = seg.size();
         while (left > 0) {
@@ -821,6 +917,11 @@ private:
     static constexpr int kMaxExplored = 128;
     static constexpr int kMaxExplored2 = 32;  // lookahead searches
     static constexpr int kLook = 4;
+    // From 12 qubits a sweep costs about a millisecond and more: a deeper
+    // lookahead (about twice the planning time) saves sweeps -- 192 -> 183 on
+    // the 2130-gate 14-qubit brick-wall circuit, 292 -> 284 on a 4000-gate
+    // 16-qubit one.
+    static constexpr int kLookDeep = 8;
     static constexpr int kWindow = 64;
     static constexpr int kMaxExploredLong = 32;  // segments above kLookMaxGates
     static constexpr size_t kLookMaxGates = 1u << 16;
@@ -828,6 +929,7 @@ private:
     const std::vector<Gate>& circuit_;
     const int n_, kmax_;
     const bool herm_;  // trailing gates must keep the block's bits a set of index bits
+    const int look_;   // candidates compared with the block that would follow them
     std::vector<std::vector<int>> lists_;  // per qubit: its gates in program order
     std::vector<int> head_;                // per qubit: first gate not yet grouped
     std::vector<uint32_t> mask_, zm_, xm_; // per circuit index: qubits, commutation classes
@@ -1036,25 +1138,48 @@ int unfused_gate_times(const std::vector<Gate>& circuit, int n, CostModel& model
 }
 
 // Predicted time of a grouping: per block the cheaper of its fused sweep (a
-// Hermitian sweep, or k_fused4 at the default width; k_fused5 for five-qubit
-// blocks) and its gates run unfused.
-int price_grouping(const std::vector<PB>& pbs, int n, bool herm, const std::vector<double>& gate_us,
-                   CostModel& model, double& total)
+// Hermitian sweep, priced on the index bits it reads the block on after the
+// relabelling of the sweeps before it; or k_fused4 at the default width;
+// k_fused5 for five-qubit blocks) and its gates run unfused.
+int price_grouping(const std::vector<Gate>& circuit, const std::vector<PB>& pbs, const Seq& seq,
+                   int n, bool herm, const std::vector<double>& gate_us, CostModel& model,
+                   double& total)
 {
     std::vector<std::vector<float>> feats;
+    std::vector<int> row(pbs.size(), -1);
     float f[kMlpFeatures];
-    for (const PB& pb : pbs) {
+    BitOrder bo(n);
+    for (const auto& e : seq) {
+        if (e.first != 0) {  // a channel: rho returns to the circuit's order first
+            bo.go_home();
+            continue;
+        }
+        const PB& pb = pbs[e.second];
         uint32_t q = pb.mask;
         for (int b = 0; b < n && __builtin_popcount(q) < 4; b++) q |= 1u << b;
         if (__builtin_popcount(q) == 5) {
             int m = default_tile_bits5(n);
             if (!tile_bits5_ok(n, m, q)) m = std::min(8, n);
             cost_features(n, CK_FUSED5, m, q, 0, -1, 0, f);
+            bo.go_home();
         } else if (herm) {
-            cost_features(n, CK_HERM, 8, q, 0, -1, 0, f);
+            // priced on the index bits the sweep reads the block on, after
+            // the relabelling of the Hermitian sweeps before it
+            uint32_t Lp[16];
+            trailing_linear(circuit, pb, n, Lp);
+            uint32_t qphys = 0;
+            BitOrder nb(n);
+            if (herm_step(n, bo, q, Lp, &qphys, &nb)) {
+                cost_features(n, CK_HERM, 8, qphys, 0, -1, 0, f);
+                bo = nb;
+            } else {
+                cost_features(n, CK_FUSED, default_tile_bits(n), q, 0, -1, 0, f);
+                bo.go_home();
+            }
         } else {
             cost_features(n, CK_FUSED, default_tile_bits(n), q, 0, -1, 0, f);
         }
+        row[e.second] = (int)feats.size();
         feats.emplace_back(f, f + kMlpFeatures);
     }
     std::vector<float> t;
@@ -1062,9 +1187,11 @@ int price_grouping(const std::vector<PB>& pbs, int n, bool herm, const std::vect
     if (rc) return rc;
     total = 0;
     for (size_t b = 0; b < pbs.size(); b++) {
+        if (row[b] < 0) continue;
+        const double fused = (double)t[row[b]];
         double unfused = 0;
         for (int gi : pbs[b].gates) unfused += gate_us[gi];
-        total += __builtin_popcount(pbs[b].mask) > 4 ? (double)t[b] : std::min((double)t[b], unfused);
+        total += __builtin_popcount(pbs[b].mask) > 4 ? fused : std::min(fused, unfused);
     }
     return DMX_OK;
 }
@@ -1079,21 +1206,32 @@ int choose_grouping(const std::vector<Gate>& circuit, int n, int kmax, bool allo
     std::vector<double> gate_us;
     int rc = unfused_gate_times(circuit, n, model, gate_us);
     if (rc) return rc;
-    // third candidate: the frontier search scoring each block by the predicted
-    // unfused time of the gates it takes (hundredths of a microsecond)
-    std::vector<long> w(circuit.size(), 1);
-    for (size_t gi = 0; gi < circuit.size(); gi++)
-        w[gi] = std::max<long>(1, std::lround(std::min(gate_us[gi], 1e9) * 100.0));
-    std::vector<PB> cand[3];
-    Seq cseq[3];
+    // further candidates: the frontier search scoring each block by the
+    // predicted unfused time of the gates it takes (hundredths of a
+    // microsecond), by that time for its multi-qubit gates only (one-qubit
+    // gates join almost any block later), and by its square root (a milder
+    // spread).  The search is greedy, so slightly different weights can give
+    // groupings some sweeps apart; the network picks among them.
+    std::vector<long> w(circuit.size(), 1), wm(circuit.size(), 1), ws(circuit.size(), 1);
+    for (size_t gi = 0; gi < circuit.size(); gi++) {
+        const double us = std::min(gate_us[gi], 1e9);
+        w[gi] = std::max<long>(1, std::lround(us * 100.0));
+        wm[gi] = __builtin_popcount(gate_mask(circuit[gi])) > 1 ? w[gi] : 1;
+        ws[gi] = std::max<long>(1, std::lround(std::sqrt(us) * 100.0));
+    }
+    constexpr int kCands = 5;
+    std::vector<PB> cand[kCands];
+    Seq cseq[kCands];
     group_least_growth(circuit, kmax, cand[0], cseq[0]);
     group_frontier(circuit, n, kmax, allow_perm, herm, cand[1], cseq[1]);
     group_frontier(circuit, n, kmax, allow_perm, herm, cand[2], cseq[2], &w);
+    group_frontier(circuit, n, kmax, allow_perm, herm, cand[3], cseq[3], &wm);
+    group_frontier(circuit, n, kmax, allow_perm, herm, cand[4], cseq[4], &ws);
     double best = 0;
     which = 0;
-    for (uint32_t c = 0; c < 3; c++) {
+    for (uint32_t c = 0; c < kCands; c++) {
         double t = 0;
-        rc = price_grouping(cand[c], n, herm, gate_us, model, t);
+        rc = price_grouping(circuit, cand[c], cseq[c], n, herm, gate_us, model, t);
         if (rc) return rc;
         if (c == 0 || t < best ||
             (t == best && grouping_cost(cand[c]) < grouping_cost(cand[which]))) {
@@ -1117,7 +1255,27 @@ int choose_executions(const std::vector<Gate>& circuit, int n, BlockPlan& bp, Co
                       std::vector<std::vector<PrimOp>>& plain, PlanInfo& info, bool herm)
 {
     const int nb = (int)bp.blocks.size();
-    // Hermitian plans: one fused option per block, the Hermitian sweep at width 8
+    // Hermitian plans: one fused option per block, the Hermitian sweep at width
+    // 8 priced on the index bits it reads the block on (hq, following the
+    // relabelling along the plan; 0 for a block that runs in order)
+    std::vector<uint32_t> hq(nb, 0);
+    if (herm) {
+        BitOrder bo(n);
+        for (const auto& o : bp.order) {
+            if (o.first != 0) {
+                bo.go_home();
+                continue;
+            }
+            const FusedBlock& fb = bp.blocks[o.second];
+            uint32_t Lp[16];
+            for (int q = 0; q < 16; q++) Lp[q] = q < n ? (fb.perm_on ? fb.perm_L[q] : 1u << q) : 0u;
+            BitOrder next(n);
+            if (herm_step(n, bo, fb.qmask, Lp, &hq[o.second], &next))
+                bo = next;
+            else
+                bo.go_home();
+        }
+    }
     const int m_hi = herm ? 4 : std::min(8, n);
     const int n_m = m_hi - 4 + 1;
     std::vector<std::vector<float>> feats;
@@ -1128,8 +1286,10 @@ int choose_executions(const std::vector<Gate>& circuit, int n, BlockPlan& bp, Co
     for (int b = 0; b < nb; b++) {
         first[b] = feats.size();
         for (int m = 4; m <= m_hi; m++) {
-            if (herm) {
-                cost_features(n, CK_HERM, 8, bp.blocks[b].qmask, 0, -1, 0, f);
+            if (herm && hq[b]) {
+                cost_features(n, CK_HERM, 8, hq[b], 0, -1, 0, f);
+            } else if (herm) {
+                cost_features(n, CK_FUSED, default_tile_bits(n), bp.blocks[b].qmask, 0, -1, 0, f);
             } else {
                 // an inadmissible width is priced like a valid one and never chosen
                 const int mm = tile_bits_ok(n, m, bp.blocks[b].qmask) ? m : m_hi;
@@ -1261,13 +1421,7 @@ int choose_fused5(const std::vector<Gate>& circuit, int n, BlockPlan& bp, CostMo
 void to_hermitian(Plan& plan, int n, PlanInfo& info)
 {
     std::vector<Step> steps;
-    std::vector<int> sigma(n), sinv(n);
-    for (int q = 0; q < n; q++) sigma[q] = sinv[q] = q;
-    auto at_home = [&] {
-        for (int q = 0; q < n; q++)
-            if (sigma[q] != q) return false;
-        return true;
-    };
+    BitOrder bo(n);
     auto move_bits = [&](const std::vector<int>& to, uint32_t v) {
         uint32_t o = 0;
         for (int q = 0; q < n; q++)
@@ -1275,15 +1429,15 @@ void to_hermitian(Plan& plan, int n, PlanInfo& info)
         return o;
     };
     auto go_home = [&] {
-        if (at_home()) return;
+        if (bo.home()) return;
         FusedBlock fb;
         std::memset(&fb, 0, sizeof(fb));
         fb.perm_on = 1;
-        for (int p = 0; p < n; p++) fb.perm_L[p] = 1u << sinv[p];  // bit p holds qubit sinv[p]
+        for (int p = 0; p < n; p++) fb.perm_L[p] = 1u << bo.sinv[p];  // bit p holds qubit sinv[p]
         steps.push_back({ST_RELABEL, (int)plan.blocks.size(), 0});
         plan.blocks.push_back(fb);
         info.relabels++;
-        for (int q = 0; q < n; q++) sigma[q] = sinv[q] = q;
+        bo.go_home();
     };
     for (const Step& st : plan.steps) {
         if (st.type != ST_FUSED4) {
@@ -1295,37 +1449,16 @@ void to_hermitian(Plan& plan, int n, PlanInfo& info)
         uint32_t Lp[16] = {};
         for (int q = 0; q < n; q++) Lp[q] = b.perm_on ? b.perm_L[q] : 1u << q;
         const uint32_t bp = b.perm_on ? b.perm_b : 0u;
-        uint32_t img = 0;
-        for (uint32_t m = b.qmask; m; m &= m - 1) img |= Lp[__builtin_ctz(m)];
-        if (__builtin_popcount(img) != 4) {  // pi(span Q) is no set of bits: in-order kernel
+        uint32_t qphys = 0;
+        BitOrder nb(n);
+        if (!herm_step(n, bo, b.qmask, Lp, &qphys, &nb)) {  // in-order kernel
             go_home();
             steps.push_back(st);
             continue;
         }
-        const uint32_t qphys = move_bits(sigma, b.qmask);
-        uint32_t cm = qphys;
-        for (int q = 0; q < n && __builtin_popcount(cm) < 8; q++) cm |= 1u << q;
-        std::vector<int> order;
-        std::vector<char> placed(n, 0);
-        auto place = [&](int q) {
-            if (!placed[q]) {
-                placed[q] = 1;
-                order.push_back(q);
-            }
-        };
-        for (int q = 0; q < n; q++)
-            if ((img >> q) & 1u) place(q);
-        for (int p = 0; p < n; p++)
-            if (((cm & ~qphys) >> p) & 1u) {
-                const uint32_t t = Lp[sinv[p]];
-                if (__builtin_popcount(t) == 1) place(__builtin_ctz(t));
-            }
-        for (int p = 0; p < n; p++) place(sinv[p]);
-        std::vector<int> so(n), soinv(n);
-        for (int i = 0; i < n; i++) {
-            so[order[i]] = i;
-            soinv[i] = order[i];
-        }
+        const std::vector<int>& sigma = bo.sigma;
+        const std::vector<int>& sinv = bo.sinv;
+        const std::vector<int>& so = nb.sigma;
         FusedBlock h;
         std::memset(&h, 0, sizeof(h));
         h.qmask = qphys;
@@ -1352,8 +1485,7 @@ void to_hermitian(Plan& plan, int n, PlanInfo& info)
         steps.push_back({ST_HERM, (int)plan.blocks.size(), 0});
         plan.blocks.push_back(h);
         info.herm_blocks++;
-        sigma = so;
-        sinv = soinv;
+        bo = nb;
     }
     go_home();
     // a Hermitian sweep followed by another writes only what that one reads

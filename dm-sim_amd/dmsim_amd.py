@@ -377,7 +377,8 @@ class Sim:
         _check(load().dmx_plan_sweeps(self._h, C.byref(pb), C.byref(hb), C.byref(rl),
                                       C.byref(gr)))
         return {"perm_blocks": pb.value, "herm_blocks": hb.value, "relabels": rl.value,
-                "grouping": ("least_growth", "frontier", "frontier_timed")[gr.value]}
+                "grouping": ("least_growth", "frontier", "frontier_timed", "frontier_timed_multi",
+                             "frontier_timed_sqrt")[gr.value]}
 
     def time_launch(self, kind, m=0, qmask=0, target=0, control=-1, slow=0, reps=10):
         """Average device microseconds of one launch configuration (rho is left
