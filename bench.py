@@ -256,7 +256,7 @@ def cpu_baseline(table, n, stride=8):
     }
 
 
-def batched(steps, warmup, seed=4, n_circuits=256):
+def batched(steps, warmup, seed=4, n_circuits=256, trace=False):
     """The batched configuration: 256 hardware-efficient-ansatz circuits
     (n uniform in 6..11, depth 10..40, RY/RZ layers + CX ladder), 32 random
     Pauli strings each; one run = every circuit from |0><0| plus every energy,
@@ -274,9 +274,11 @@ def batched(steps, warmup, seed=4, n_circuits=256):
                     f"seed {seed}",
         "gates": gates,
     }
-    for tuned in (True, False):
+    for tuned in ((True,) if trace else (True, False)):
         b = dmsim_amd.Batch()
         b.set_autotune(tuned)
+        if trace:  # eager launches, which a kernel-trace profiler records one by one
+            b.set_graph(False)
         for circ, strings, weights in items:
             b.add_circuit(circ, strings, weights)
         b.upload()
@@ -305,6 +307,8 @@ def batched(steps, warmup, seed=4, n_circuits=256):
                 "note": "rho of circuits up to 11 qubits fits the 256 MB Infinity Cache",
             }
         b.close()
+    if trace:
+        return out
     for mname in ("plain", "plain_launch", "fused"):
         sims = {}
         dt = 0.0
@@ -419,6 +423,9 @@ def main():
     ap.add_argument("--no-large", action="store_true",
                     help="skip the n = 16 learned-vs-greedy and n = 15 structured x16 lines")
     ap.add_argument("--calibrate-copy", action="store_true")
+    ap.add_argument("--batch-trace", action="store_true",
+                    help="only the batched configuration, autotuned, launched eagerly "
+                         "(for a kernel-trace profiler); prints its JSON line")
     ap.add_argument("--gates", default="",
                     help="LO:HI -- run only gates LO..HI-1 of the workload (profiling runs)")
     args = ap.parse_args()
@@ -428,6 +435,9 @@ def main():
         sys.exit(2)
     if args.calibrate_copy:
         calibrate_copy()
+        return
+    if args.batch_trace:
+        print(json.dumps(batched(args.steps, args.warmup, trace=True)), flush=True)
         return
 
     import dmsim_amd
