@@ -12,6 +12,7 @@
 
 #include <algorithm>
 #include <map>
+#include <tuple>
 
 #include "dmsim_amd.h"
 #include "engine.hpp"
@@ -75,8 +76,7 @@ uint64_t Batch::gates() const
 double Batch::sweep_bytes() const
 {
     double b = 0;
-    for (const Circ& c : circs_)
-        b += c.plan ? c.plan->bytes(c.n) : (double)c.n_blocks * 32.0 * (double)(1ull << (2 * c.n));
+    for (const Circ& c : circs_) b += c.bytes;
     return b;
 }
 
@@ -213,8 +213,11 @@ int Batch::upload()
     std::vector<uint64_t> re_off;
     uint64_t off = 0;
     bool any_perm = false;
+    std::vector<uint8_t> kinds;
     for (Circ& c : circs_) {
         c.blk_off = blocks.size();
+        c.bytes = 0;
+        c.herm = false;
         c.n_blocks = 0;
         if (c.n >= kSoloQubits) {
             // planned as the single-circuit engine plans it alone (fused
@@ -241,7 +244,40 @@ int Batch::upload()
                     (v.kind == PlanStepView::FUSED4 && v.block->perm_on))
                     alt = !alt;
             c.final_alt = alt;
+            c.bytes = p->bytes(c.n);
             continue;
+        }
+        const double sweep = 32.0 * (double)(1ull << (2 * c.n));
+        if (hermitian_ && c.n >= kBatchHermQubits) {
+            // Hermitian sweeps and relabellings, planned as the single-circuit
+            // engine plans with Hermitian sweeps from 8 qubits
+            FusedPlan* p = nullptr;
+            PlanInfo pi;
+            int rc = build_fused_plan(c.gates, c.n, 4, model_, 0, true, &p, &pi, true);
+            if (rc) return rc;
+            std::unique_ptr<FusedPlan> hold(p);
+            std::vector<PlanStepView> steps;
+            bool ok = p != nullptr;
+            if (ok) {
+                p->describe(steps);
+                for (const PlanStepView& v : steps)
+                    ok = ok && (v.kind == PlanStepView::HERM || v.kind == PlanStepView::RELABEL ||
+                                v.kind == PlanStepView::FUSED4);
+            }
+            if (ok) {
+                for (const PlanStepView& v : steps) {
+                    blocks.push_back(*v.block);
+                    const uint8_t k = v.kind == PlanStepView::HERM      ? SK_HERM
+                                      : v.kind == PlanStepView::RELABEL ? SK_RELABEL
+                                                                        : SK_INORDER;
+                    kinds.push_back(k);
+                    c.bytes += k == SK_HERM ? herm_bytes(c.n, v.block->next_q != 0) : sweep;
+                }
+                c.n_blocks = steps.size();
+                c.herm = true;
+                continue;
+            }
+            // a step that is no batched sweep (an unfused segment): in-order blocks
         }
         // the single-circuit engine's choice (same planner, same rule), so a
         // batched rho is bit-identical to the circuit's own fused run
@@ -271,7 +307,10 @@ int Batch::upload()
             }
         c.n_blocks = bp.blocks.size();
         blocks.insert(blocks.end(), bp.blocks.begin(), bp.blocks.end());
+        kinds.insert(kinds.end(), bp.blocks.size(), (uint8_t)SK_INORDER);
+        c.bytes = (double)c.n_blocks * sweep;
     }
+    kinds_ = kinds;
     for (const FusedBlock& fb : blocks) any_perm = any_perm || fb.perm_on;
     // waves: consecutive circuits whose rho fit one pool (and its twin for
     // permuting blocks); every wave reuses the pool from offset 0
@@ -395,13 +434,19 @@ int Batch::upload()
                 size_t steps = 0;
                 for (int i : grp) steps = std::max(steps, circs_[i].n_blocks);
                 for (size_t s = 0; s < steps; s++) {
-                    std::map<std::pair<int, int>, std::vector<int>> by;
+                    // per (qubit count, step kind, tile width)
+                    std::map<std::tuple<int, int, int>, std::vector<int>> by;
                     for (int i : grp)
-                        if (s < circs_[i].n_blocks)
-                            by[{circs_[i].n, (int)blocks[circs_[i].blk_off + s].m}].push_back(i);
+                        if (s < circs_[i].n_blocks) {
+                            const size_t b = circs_[i].blk_off + s;
+                            const int kind = kinds_[b];
+                            const int m = kind == SK_INORDER ? (int)blocks[b].m : 8;
+                            by[{circs_[i].n, kind, m}].push_back(i);
+                        }
                     for (const auto& kv : by) {
-                        lst.push_back(Launch{kv.first.first, kv.first.second,
-                                             (int)kv.second.size(), ents.size(), w});
+                        lst.push_back(Launch{std::get<0>(kv.first), std::get<2>(kv.first),
+                                             (int)kv.second.size(), ents.size(), w,
+                                             std::get<1>(kv.first)});
                         for (int i : kv.second) ents.push_back(entry(i, s));
                     }
                 }
@@ -494,9 +539,10 @@ int Batch::tile_bits(int n) const
 // reset; the sweeps act on whatever the pool holds.
 int Batch::autotune(std::vector<FusedBlock>& blocks)
 {
-    std::map<int, std::vector<int>> by_n;
+    std::map<int, std::vector<int>> by_n;  // circuits of in-order blocks by qubit count
     for (int i = 0; i < (int)circs_.size(); i++)
-        if (circs_[i].n_blocks > 0) by_n[circs_[i].n].push_back(i);
+        if (circs_[i].n_blocks > 0 && !circs_[i].herm) by_n[circs_[i].n].push_back(i);
+    if (by_n.empty()) return DMX_OK;
     BTRY(hipMemsetAsync(pool_, 0, pool_doubles_ * sizeof(double), stream_));
     size_t most = 0;
     for (const auto& kv : by_n) most = std::max(most, kv.second.size());
@@ -544,7 +590,9 @@ int Batch::autotune(std::vector<FusedBlock>& blocks)
     (void)hipFree(d_tmp);
     if (rc) return rc;
     for (const Circ& c : circs_)
-        for (size_t b = 0; b < c.n_blocks; b++) blocks[c.blk_off + b].m = (uint32_t)tile_bits_[c.n];
+        if (!c.herm)
+            for (size_t b = 0; b < c.n_blocks; b++)
+                blocks[c.blk_off + b].m = (uint32_t)tile_bits_[c.n];
     return DMX_OK;
 }
 
@@ -580,8 +628,7 @@ int Batch::autotune_groups(const LaunchBuilder& build)
         for (int rep = 0; rep < 2 && e == hipSuccess; rep++) {  // rep 0 warms up
             if (rep == 1) e = hipEventRecord(ev0_, stream_);
             for (const Launch& l : lsts[c])
-                if (e == hipSuccess)
-                    e = launch_fused4_batch(d_tmp + l.entry_off, l.count, l.n, l.m, 4096, stream_);
+                if (e == hipSuccess) e = launch(l, d_tmp);
         }
         if (e == hipSuccess) e = hipEventRecord(ev1_, stream_);
         if (e == hipSuccess) e = hipEventSynchronize(ev1_);
@@ -601,15 +648,33 @@ int Batch::autotune_groups(const LaunchBuilder& build)
     return DMX_OK;
 }
 
+hipError_t Batch::launch(const Launch& l, const BatchEntry* d_entries)
+{
+    const BatchEntry* e = d_entries + l.entry_off;
+    switch (l.kind) {
+    case SK_HERM: return launch_fused4_herm_batch(e, l.count, l.n, 4096, stream_);
+    case SK_RELABEL: return launch_relabel_batch(e, l.count, l.n, 4096, stream_);
+    default: return launch_fused4_batch(e, l.count, l.n, l.m, 4096, stream_);
+    }
+}
+
+int Batch::set_hermitian(bool on)
+{
+    if (uploaded_) {
+        set_error("set_hermitian after upload");
+        return DMX_ESTATE;
+    }
+    hermitian_ = on;
+    return DMX_OK;
+}
+
 int Batch::enqueue()
 {
     for (int w = 0; w < (int)waves(); w++) {
         const int first = wave_first_[w], count = wave_count_[w];
         BTRY(launch_pool_reset(pool_, wave_doubles_[w], d_re_off_ + first, count, stream_));
         for (const Launch& l : launches_)
-            if (l.wave == w)
-                BTRY(launch_fused4_batch(d_entries_ + l.entry_off, l.count, l.n, l.m, 4096,
-                                         stream_));
+            if (l.wave == w) BTRY(launch(l, d_entries_));
         for (int i = first; i < first + count; i++) {
             Circ& c = circs_[i];
             if (!c.solo) continue;

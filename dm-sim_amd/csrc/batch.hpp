@@ -88,6 +88,15 @@ public:
     // (Hermitian block sweeps, five-qubit gates kept whole), instead of in
     // the batched block launches.
     static constexpr int kSoloQubits = kHermAutoQubits;
+    // Circuits of kBatchHermQubits..kSoloQubits-1 qubits run Hermitian block
+    // sweeps (batched k_fused4_herm / k_relabel launches; planned as the
+    // single-circuit engine plans with dmx_set_fusion_hermitian(s, 2)) when on
+    // (the default); off: in-order blocks.  Measured on 43 hardware-efficient
+    // circuits of one width, Hermitian against in-order: 0.75x the time at 11
+    // qubits, 0.98x at 10, 1.26x at 9 and 1.58x at 8, whose sweeps have too
+    // few items to fill the GPU.
+    int set_hermitian(bool on);
+    static constexpr int kBatchHermQubits = 10;
 
     double last_ms() const { return last_ms_; }
     uint32_t launches() const
@@ -110,6 +119,8 @@ private:
         std::vector<Gate> orig;   // as added
         std::vector<Gate> gates;  // C4X expanded (batched blocks have at most four qubits)
         bool solo = false;        // runs on its own (kSoloQubits and up)
+        bool herm = false;        // its batched steps are Hermitian sweeps and relabellings
+        double bytes = 0;         // algorithmic bytes of its steps
         std::unique_ptr<FusedPlan> plan;  // its plan then
         uint32_t solo_launches = 0;
         std::vector<PauliTerm> paulis;
@@ -120,13 +131,16 @@ private:
         int wave = 0;
         uint64_t diag_off = 0;  // its diagonal in the kept diagonals (several waves)
     };
+    enum StepKind : uint8_t { SK_INORDER = 0, SK_HERM = 1, SK_RELABEL = 2 };
     struct Launch {
         int n;
         int m;
         int count;
         size_t entry_off;
         int wave;
+        int kind;  // StepKind
     };
+    hipError_t launch(const Launch& l, const BatchEntry* d_entries);
     using LaunchBuilder =
         std::function<size_t(uint64_t, std::vector<BatchEntry>&, std::vector<Launch>&)>;
     int enqueue();  // the whole run on stream_ (captured or eager)
@@ -137,6 +151,8 @@ private:
     std::vector<Circ> circs_;
     std::vector<Launch> launches_;
     CostModel* model_ = nullptr;
+    bool hermitian_ = true;
+    std::vector<uint8_t> kinds_;  // StepKind of every batched step (parallel to the blocks)
     LaunchProfiler quiet_;  // disabled: the solo plans' executor times nothing
     bool uploaded_ = false;
     bool use_graph_ = true;

@@ -667,6 +667,12 @@ int dmx_batch_schedule(const dmx_batch* b, uint64_t* group_bytes, uint32_t* grou
     return DMX_OK;
 }
 
+int dmx_batch_set_hermitian(dmx_batch* b, int on)
+{
+    if (!b) return null_handle();
+    return b->b.set_hermitian(on != 0);
+}
+
 int dmx_batch_set_group_bytes(dmx_batch* b, uint64_t bytes)
 {
     if (!b) return null_handle();

@@ -63,6 +63,22 @@ __device__ __forceinline__ void st_stream(double* p, dv2 v)
     __builtin_nontemporal_store(v, reinterpret_cast<dv2*>(p));
 }
 
+template <bool NT>
+__device__ __forceinline__ dv2 ld_v2(const double* p)
+{
+    if constexpr (NT) return ld_stream(p);
+    return *reinterpret_cast<const dv2*>(p);
+}
+
+template <bool NT>
+__device__ __forceinline__ void st_v2(double* p, dv2 v)
+{
+    if constexpr (NT)
+        st_stream(p, v);
+    else
+        *reinterpret_cast<dv2*>(p) = v;
+}
+
 // Scatter the low bits of v into the positions of `mask` (ascending).
 __device__ __forceinline__ uint64_t deposit(uint64_t v, uint64_t mask)
 {
@@ -334,23 +350,24 @@ __global__ void __launch_bounds__(256, 2) k_fused4_batch(const BatchEntry* __res
 // which the host chooses so that every tile lands in 16 rows x one aligned
 // run of 16 columns both ways (index maps: herm.hpp).  phi also carries the
 // block's trailing X/CX/SWAP gates and the new bit order.
-__global__ void __launch_bounds__(256, 2) k_fused4_herm(const double* __restrict__ re,
-                                                         const double* __restrict__ im,
-                                                         double* __restrict__ re_out,
-                                                         double* __restrict__ im_out,
-                                                         const FusedBlock* __restrict__ blk,
-                                                         int n)
+// One Hermitian block sweep over the items `next` deals this workgroup:
+// next(k, w) sets w to its k-th item and returns false once they are used up
+// (then for every later k as well).  NT: nontemporal loads and stores (rho
+// streamed from HBM); without, cached ones (rho in the Infinity Cache).
+template <bool NT, bool PIPE, class Next>
+__device__ __forceinline__ void herm_sweep(const double* __restrict__ re,
+                                           const double* __restrict__ im,
+                                           double* __restrict__ re_out,
+                                           double* __restrict__ im_out,
+                                           const FusedBlock* __restrict__ blk, const HermGeom& g,
+                                           Next next, double* __restrict__ s_re,
+                                           double* __restrict__ s_im)
 {
-    __shared__ double s_re[16 * kHermPitch];
-    __shared__ double s_im[16 * kHermPitch];
     const int tid = threadIdx.x;
     const int lane = tid & 63;
     const int wave = tid >> 6;
     const int j = lane & 15;
     const int kk = lane >> 4;
-
-    HermGeom g;
-    g.init(*blk, n);
 
     double ur[4], ui[4], upi[4], nui[4], umi[4];
 #pragma unroll
@@ -370,24 +387,26 @@ __global__ void __launch_bounds__(256, 2) k_fused4_herm(const double* __restrict
     HermThread th;
     th.init(g, tid);
     const int lc = th.lc;
-    // Software pipeline: the next item's loads are issued before this item's
-    // transform and stores and land in registers while they run.  rho is
-    // streamed once per sweep: loads and stores are nontemporal.
+    // PIPE: software pipeline, the next item's loads are issued before this
+    // item's transform and stores and land in registers while they run;
+    // without, each item is loaded when its turn comes (fewer registers).
     dv2 vr[8], vi[8];
     uint64_t w = 0;
-    bool have = herm_slot(g, blockIdx.x, gridDim.x, 0, w);
+    bool have = next(0, w);
     HermItem it;
-    if (have) {
+    auto load = [&] {
         it = herm_item(g, w);
 #pragma unroll
         for (int rep = 0; rep < 8; rep++) {
             int a;
             const uint64_t off = herm_load(g, th, it, rep, a);
-            vr[rep] = ld_stream(re + off);
-            vi[rep] = ld_stream(im + off);
+            vr[rep] = ld_v2<NT>(re + off);
+            vi[rep] = ld_v2<NT>(im + off);
         }
-    }
+    };
+    if (PIPE && have) load();
     for (uint64_t k = 0; have; k++) {
+        if (!PIPE) load();
 #pragma unroll
         for (int rep = 0; rep < 8; rep++) {
             const int a = th.top + 2 * rep;
@@ -398,17 +417,8 @@ __global__ void __launch_bounds__(256, 2) k_fused4_herm(const double* __restrict
         }
         __syncthreads();
         const HermItem cur = it;
-        have = herm_slot(g, blockIdx.x, gridDim.x, k + 1, w);
-        if (have) {
-            it = herm_item(g, w);
-#pragma unroll
-            for (int rep = 0; rep < 8; rep++) {
-                int a;
-                const uint64_t off = herm_load(g, th, it, rep, a);
-                vr[rep] = ld_stream(re + off);
-                vi[rep] = ld_stream(im + off);
-            }
-        }
+        have = next(k + 1, w);
+        if (PIPE && have) load();
 
         for (int tb = wave; tb < 16; tb += 4) {
             if (!herm_tile_on(cur, tb)) continue;
@@ -453,8 +463,8 @@ __global__ void __launch_bounds__(256, 2) k_fused4_herm(const double* __restrict
             uint64_t off;
             int a, c0, c1;
             if (herm_direct(g, th, cur, rep, off, a, c0, c1)) {
-                st_stream(re_out + off, dv2{s_re[herm_lds(a, c0)], s_re[herm_lds(a, c1)]});
-                st_stream(im_out + off, dv2{s_im[herm_lds(a, c0)], s_im[herm_lds(a, c1)]});
+                st_v2<NT>(re_out + off, dv2{s_re[herm_lds(a, c0)], s_re[herm_lds(a, c1)]});
+                st_v2<NT>(im_out + off, dv2{s_im[herm_lds(a, c0)], s_im[herm_lds(a, c1)]});
             }
         }
 #pragma unroll
@@ -462,12 +472,50 @@ __global__ void __launch_bounds__(256, 2) k_fused4_herm(const double* __restrict
             uint64_t off;
             int a0, a1, col;
             if (herm_mirror(g, th, cur, rep, off, a0, a1, col)) {
-                st_stream(re_out + off, dv2{s_re[herm_lds(a0, col)], s_re[herm_lds(a1, col)]});
-                st_stream(im_out + off, dv2{-s_im[herm_lds(a0, col)], -s_im[herm_lds(a1, col)]});
+                st_v2<NT>(re_out + off, dv2{s_re[herm_lds(a0, col)], s_re[herm_lds(a1, col)]});
+                st_v2<NT>(im_out + off, dv2{-s_im[herm_lds(a0, col)], -s_im[herm_lds(a1, col)]});
             }
         }
         __syncthreads();
     }
+}
+
+__global__ void __launch_bounds__(256, 2) k_fused4_herm(const double* __restrict__ re,
+                                                         const double* __restrict__ im,
+                                                         double* __restrict__ re_out,
+                                                         double* __restrict__ im_out,
+                                                         const FusedBlock* __restrict__ blk,
+                                                         int n)
+{
+    __shared__ double s_re[16 * kHermPitch];
+    __shared__ double s_im[16 * kHermPitch];
+    HermGeom g;
+    g.init(*blk, n);
+    herm_sweep<true, true>(re, im, re_out, im_out, blk, g,
+                     [&](uint64_t k, uint64_t& w) {
+                         return herm_slot(g, blockIdx.x, gridDim.x, k, w);
+                     },
+                     s_re, s_im);
+}
+
+// Batched form: blockIdx.y selects one circuit's (rho, block) pair, all of
+// qubit count n; its items are dealt grid-stride over gridDim.x workgroups.
+// Cached loads and stores (the batch's circuits are below 12 qubits).
+__global__ void __launch_bounds__(256, 2) k_fused4_herm_batch(const BatchEntry* __restrict__ entries,
+                                                               int n)
+{
+    __shared__ double s_re[16 * kHermPitch];
+    __shared__ double s_im[16 * kHermPitch];
+    const BatchEntry e = entries[blockIdx.y];
+    HermGeom g;
+    g.init(*e.blk, n);
+    const uint64_t items = g.items();
+    herm_sweep<false, false>(e.re, e.im, e.re_out, e.im_out, e.blk, g,
+                      [&](uint64_t k, uint64_t& w) {
+                          w = blockIdx.x + k * gridDim.x;
+                          return w < items;
+                      },
+                      s_re, s_im);
 }
 
 // rho's index bits relabelled, entry (x, y) -> (p(x), p(y)) for a bit
@@ -475,14 +523,14 @@ __global__ void __launch_bounds__(256, 2) k_fused4_herm(const double* __restrict
 // 16 consecutive rows x 256 columns whose column bits hold bits 0..3 and the
 // bits p maps to 0..3, so reads and writes move runs of at least 16 doubles
 // (index maps: herm.hpp).
-__global__ void __launch_bounds__(256, 2) k_relabel(const double* __restrict__ re,
-                                                     const double* __restrict__ im,
-                                                     double* __restrict__ re_out,
-                                                     double* __restrict__ im_out,
-                                                     const FusedBlock* __restrict__ blk, int n)
+template <bool NT>
+__device__ __forceinline__ void relabel_sweep(const double* __restrict__ re,
+                                              const double* __restrict__ im,
+                                              double* __restrict__ re_out,
+                                              double* __restrict__ im_out,
+                                              const FusedBlock* __restrict__ blk, int n,
+                                              double* __restrict__ s_re, double* __restrict__ s_im)
 {
-    __shared__ double s_re[16 * kHermPitch];
-    __shared__ double s_im[16 * kHermPitch];
     const int tid = threadIdx.x;
     RelabelGeom g;
     g.init(*blk, n);
@@ -496,8 +544,8 @@ __global__ void __launch_bounds__(256, 2) k_relabel(const double* __restrict__ r
         for (int rep = 0; rep < 8; rep++) {
             int a;
             const uint64_t off = relabel_load(g, th, it, tid, rep, a);
-            const dv2 vr = ld_stream(re + off);
-            const dv2 vi = ld_stream(im + off);
+            const dv2 vr = ld_v2<NT>(re + off);
+            const dv2 vi = ld_v2<NT>(im + off);
             s_re[a * kHermPitch + lc] = vr.x;
             s_re[a * kHermPitch + lc + 1] = vr.y;
             s_im[a * kHermPitch + lc] = vi.x;
@@ -508,11 +556,32 @@ __global__ void __launch_bounds__(256, 2) k_relabel(const double* __restrict__ r
         for (int rep = 0; rep < 8; rep++) {
             int a;
             const uint64_t off = relabel_store(g, th, it, tid, rep, a);
-            st_stream(re_out + off, dv2{s_re[a * kHermPitch + th.s0], s_re[a * kHermPitch + th.s1]});
-            st_stream(im_out + off, dv2{s_im[a * kHermPitch + th.s0], s_im[a * kHermPitch + th.s1]});
+            st_v2<NT>(re_out + off, dv2{s_re[a * kHermPitch + th.s0], s_re[a * kHermPitch + th.s1]});
+            st_v2<NT>(im_out + off, dv2{s_im[a * kHermPitch + th.s0], s_im[a * kHermPitch + th.s1]});
         }
         __syncthreads();
     }
+}
+
+__global__ void __launch_bounds__(256, 2) k_relabel(const double* __restrict__ re,
+                                                     const double* __restrict__ im,
+                                                     double* __restrict__ re_out,
+                                                     double* __restrict__ im_out,
+                                                     const FusedBlock* __restrict__ blk, int n)
+{
+    __shared__ double s_re[16 * kHermPitch];
+    __shared__ double s_im[16 * kHermPitch];
+    relabel_sweep<true>(re, im, re_out, im_out, blk, n, s_re, s_im);
+}
+
+// Batched form: blockIdx.y selects one circuit's (rho, relabelling) pair.
+__global__ void __launch_bounds__(256, 2) k_relabel_batch(const BatchEntry* __restrict__ entries,
+                                                           int n)
+{
+    __shared__ double s_re[16 * kHermPitch];
+    __shared__ double s_im[16 * kHermPitch];
+    const BatchEntry e = entries[blockIdx.y];
+    relabel_sweep<false>(e.re, e.im, e.re_out, e.im_out, e.blk, n, s_re, s_im);
 }
 
 namespace {
@@ -599,6 +668,31 @@ hipError_t launch_relabel(const double* re, const double* im, double* re_out, do
     const uint64_t items = (1ull << (n - 4)) * (1ull << (n - 8));
     const unsigned blocks = (unsigned)(items < (uint64_t)max_blocks ? items : (uint64_t)max_blocks);
     k_relabel<<<blocks, 256, 0, stream>>>(re, im, re_out, im_out, d_blk, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_fused4_herm_batch(const BatchEntry* d_entries, int count, int n,
+                                    int max_blocks, hipStream_t stream)
+{
+    if (n < 8 || n > 16 || count < 1 || count > 65535) return hipErrorInvalidValue;
+    const uint64_t H = 1ull << (n - 8);
+    const uint64_t items = 16 * (H * (H + 1) / 2);
+    uint64_t per = ((uint64_t)max_blocks + count - 1) / count;
+    if (per < 1) per = 1;
+    const dim3 grid((unsigned)(items < per ? items : per), (unsigned)count);
+    k_fused4_herm_batch<<<grid, 256, 0, stream>>>(d_entries, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_relabel_batch(const BatchEntry* d_entries, int count, int n, int max_blocks,
+                                hipStream_t stream)
+{
+    if (n < 8 || n > 16 || count < 1 || count > 65535) return hipErrorInvalidValue;
+    const uint64_t items = (1ull << (n - 4)) * (1ull << (n - 8));
+    uint64_t per = ((uint64_t)max_blocks + count - 1) / count;
+    if (per < 1) per = 1;
+    const dim3 grid((unsigned)(items < per ? items : per), (unsigned)count);
+    k_relabel_batch<<<grid, 256, 0, stream>>>(d_entries, n);
     return hipGetLastError();
 }
 

@@ -98,6 +98,13 @@ hipError_t launch_relabel(const double* re, const double* im, double* re_out, do
 
 // The same for `count` circuits of n qubits each (one launch of about
 // max_blocks workgroups in all).
+// Batched Hermitian block sweeps and relabellings (n = 8..16): entry c of
+// d_entries is circuit c's (rho, second planes, block); about max_blocks
+// workgroups in all, each looping over items of its circuit.  Out of place.
+hipError_t launch_fused4_herm_batch(const BatchEntry* d_entries, int count, int n,
+                                    int max_blocks, hipStream_t stream);
+hipError_t launch_relabel_batch(const BatchEntry* d_entries, int count, int n, int max_blocks,
+                                hipStream_t stream);
 hipError_t launch_fused4_batch(const BatchEntry* d_entries, int count, int n, int m,
                                int max_blocks, hipStream_t stream);
 

@@ -41,7 +41,7 @@ EXPORTS = [
     "dmx_batch_run", "dmx_batch_energies", "dmx_batch_get_dm", "dmx_batch_stats",
     "dmx_batch_sweep_bytes", "dmx_batch_get_diag", "dmx_batch_set_pool_limit",
     "dmx_batch_waves", "dmx_batch_set_cost_model", "dmx_batch_schedule",
-    "dmx_batch_set_group_bytes",
+    "dmx_batch_set_group_bytes", "dmx_batch_set_hermitian",
     "dmx_last_error",
 ]
 
@@ -134,6 +134,7 @@ def load():
     lib.dmx_batch_set_cost_model.argtypes = [vp, vp]
     lib.dmx_batch_schedule.argtypes = [vp, P(C.c_uint64), P(C.c_uint32)]
     lib.dmx_batch_set_group_bytes.argtypes = [vp, C.c_uint64]
+    lib.dmx_batch_set_hermitian.argtypes = [vp, C.c_int]
     lib.dmx_batch_stats.argtypes = [vp, P(C.c_double), P(C.c_uint32), P(C.c_uint32),
                                     P(C.c_uint64)]
     lib.dmx_last_error.restype = C.c_char_p
@@ -576,6 +577,10 @@ class Batch:
         w = C.c_uint32()
         _check(load().dmx_batch_waves(self._h, C.byref(w)))
         return w.value
+
+    def set_hermitian(self, on: bool):
+        """Hermitian block sweeps for the circuits of 10 and 11 qubits (default on)."""
+        _check(load().dmx_batch_set_hermitian(self._h, 1 if on else 0))
 
     def set_group_bytes(self, nbytes: int):
         """Fixes the schedule's group budget (0 = whole waves) instead of

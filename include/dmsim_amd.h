@@ -417,6 +417,15 @@ int dmx_batch_schedule(const dmx_batch* b, uint64_t* group_bytes, uint32_t* grou
  * changes no circuit's arithmetic: results are bit-identical under any. */
 int dmx_batch_set_group_bytes(dmx_batch* b, uint64_t bytes);
 
+/* Extension: Hermitian block sweeps for the batch's circuits of 10 and 11
+ * qubits (on, the default; below 10 in-order blocks are faster in a batch):
+ * their blocks run as batched Hermitian sweeps (the tiles
+ * on and above the diagonal, each result written with its conjugate
+ * transpose) and relabellings, planned as dmx_upload plans the circuit alone
+ * with dmx_set_fusion_hermitian(s, 2); off (0): in-order blocks.  Before
+ * dmx_batch_upload. */
+int dmx_batch_set_hermitian(dmx_batch* b, int on);
+
 /* Extension: the planner network for the batch's circuits of 12 or more
  * qubits (those that run on their own; null: the greedy planner), used at
  * dmx_batch_upload.  Destroying the model detaches it. */

@@ -78,14 +78,22 @@ def test_batch_matches_oracle_dense(oracle):
         assert max(np.abs(gre - re).max(), np.abs(gim - im).max()) < 1e-10
 
 
-@pytest.mark.parametrize("autotune", [True, False])
-def test_batch_bitwise_equals_single_fused_run(autotune):
+@pytest.mark.parametrize("autotune,herm", [(True, True), (False, True), (True, False)])
+def test_batch_bitwise_equals_single_fused_run(autotune, herm):
     """Tile widths only regroup the 16 x 16 tiles: the batch, autotuned or
-    not, reproduces the single-circuit fused run bit for bit."""
+    not, reproduces the single-circuit fused run bit for bit -- with
+    Hermitian sweeps for its circuits of 10 and 11 qubits, the run planned
+    with Hermitian sweeps (dmx_set_fusion_hermitian 2), without, the in-order
+    run."""
     import dmsim_amd
     items = small_items(seed=9, count=6)
+    rng = np.random.default_rng(19)
+    for n in (8, 9, 10, 11):
+        strings, weights = W.random_paulis(n, 4, rng)
+        items.append((W.hea_circuit(n, 6, rng), strings, weights))
     b = dmsim_amd.Batch()
     b.set_autotune(autotune)
+    b.set_hermitian(herm)
     for circ, strings, weights in items:
         b.add_circuit(circ, strings, weights)
     b.upload()
@@ -97,8 +105,11 @@ def test_batch_bitwise_equals_single_fused_run(autotune):
     for cid, (circ, _, _) in enumerate(items):
         s = dmsim_amd.Sim(circ.n_qubits)
         s.set_method(dmsim_amd.METHOD_FUSED)
+        s.set_fusion_hermitian(2 if herm and circ.n_qubits >= 10 else 0)
         s.append_table(circ)
         s.upload()
+        if herm and circ.n_qubits >= 10:
+            assert s.plan_info()["herm_blocks"] > 0
         s.run()
         re, im = s.get_dm()
         bre, bim = b.get_dm(cid)
