@@ -313,3 +313,36 @@ def test_batch_cache_groups_are_bit_identical(oracle):
         assert abs(whole.energies()[cid] - sv_energy(psi, strings, weights)) < 1e-9
     for b in (whole, grouped, timed):
         b.close()
+
+
+def test_batch_hermitian_circuits_in_waves(oracle):
+    """10- and 11-qubit circuits (batched Hermitian sweeps) split over several
+    waves by a pool limit give energies and diagonals bit for bit equal to one
+    wave, and the state-vector oracle's to 1e-10."""
+    import dmsim_amd
+    rng = np.random.default_rng(1011)
+    items = []
+    for n in (10, 11, 10, 11, 10):
+        strings, weights = W.random_paulis(n, 6, rng)
+        items.append((W.hea_circuit(n, int(rng.integers(4, 10)), rng), strings, weights))
+
+    def build(limit):
+        b = dmsim_amd.Batch()
+        b.set_pool_limit(limit)
+        for circ, strings, weights in items:
+            b.add_circuit(circ, strings, weights)
+        b.upload()
+        b.run()
+        return b
+
+    one = build(0)
+    many = build(16 << 22)  # one 11-qubit state per pool: five waves
+    assert one.waves() == 1 and many.waves() == 5
+    assert np.array_equal(one.energies(), many.energies())
+    for cid, (circ, strings, weights) in enumerate(items):
+        assert np.array_equal(one.get_diag(cid), many.get_diag(cid)), cid
+        psi = oracle.sv(circ.n_qubits, circ.arrays())
+        assert np.max(np.abs(one.get_diag(cid) - np.abs(psi) ** 2)) <= 1e-10
+        assert abs(one.energies()[cid] - sv_energy(psi, strings, weights)) < 1e-9
+    one.close()
+    many.close()
