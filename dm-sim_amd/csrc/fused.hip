@@ -500,7 +500,9 @@ __global__ void __launch_bounds__(256, 2) k_fused4_herm(const double* __restrict
 
 // Batched form: blockIdx.y selects one circuit's (rho, block) pair, all of
 // qubit count n; its items are dealt grid-stride over gridDim.x workgroups.
-// Cached loads and stores (the batch's circuits are below 12 qubits).
+// Cached loads and stores (the batch's circuits are below 12 qubits); PIPE
+// as in herm_sweep.
+template <bool PIPE>
 __global__ void __launch_bounds__(256, 2) k_fused4_herm_batch(const BatchEntry* __restrict__ entries,
                                                                int n)
 {
@@ -510,7 +512,7 @@ __global__ void __launch_bounds__(256, 2) k_fused4_herm_batch(const BatchEntry* 
     HermGeom g;
     g.init(*e.blk, n);
     const uint64_t items = g.items();
-    herm_sweep<false, false>(e.re, e.im, e.re_out, e.im_out, e.blk, g,
+    herm_sweep<false, PIPE>(e.re, e.im, e.re_out, e.im_out, e.blk, g,
                       [&](uint64_t k, uint64_t& w) {
                           w = blockIdx.x + k * gridDim.x;
                           return w < items;
@@ -591,6 +593,9 @@ uint64_t supertiles(int n, int m) { return (1ull << (n - 4)) * (1ull << (n - m))
 // From this many qubits rho (>= 268 MB) streams from HBM rather than staying
 // in the Infinity Cache: the in-order sweeps pipeline their loads.
 constexpr int kPipeQubits = 12;
+
+// From this many qubits the batched Hermitian sweeps pipeline their loads.
+constexpr int kBatchHermPipeQubits = 11;
 
 }  // namespace
 
@@ -680,7 +685,10 @@ hipError_t launch_fused4_herm_batch(const BatchEntry* d_entries, int count, int 
     uint64_t per = ((uint64_t)max_blocks + count - 1) / count;
     if (per < 1) per = 1;
     const dim3 grid((unsigned)(items < per ? items : per), (unsigned)count);
-    k_fused4_herm_batch<<<grid, 256, 0, stream>>>(d_entries, n);
+    if (n >= kBatchHermPipeQubits)
+        k_fused4_herm_batch<true><<<grid, 256, 0, stream>>>(d_entries, n);
+    else
+        k_fused4_herm_batch<false><<<grid, 256, 0, stream>>>(d_entries, n);
     return hipGetLastError();
 }
 
