@@ -38,6 +38,7 @@ Batch::~Batch()
 
 void Batch::free_device()
 {
+    if (stream_) (void)hipStreamSynchronize(stream_);  // nothing in flight reads them
     if (exec_) (void)hipGraphExecDestroy(exec_);
     if (graph_) (void)hipGraphDestroy(graph_);
     exec_ = nullptr;
@@ -207,6 +208,15 @@ int Batch::upload()
     if (circs_.empty() || circs_.size() > 65535) {
         set_error("a batch needs 1..65535 circuits");
         return DMX_EINVAL;
+    }
+    // a failed earlier attempt may have left device buffers and launches
+    free_device();
+    launches_.clear();
+    kinds_.clear();
+    for (Circ& c : circs_) {
+        c.plan.reset();
+        c.solo = c.herm = false;
+        c.solo_launches = 0;
     }
     std::vector<FusedBlock> blocks;
     std::vector<PauliTerm> terms;
