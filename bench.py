@@ -244,7 +244,7 @@ def cpu_baseline(table, n, stride=8):
     t0 = time.perf_counter()
     o.sim(n, sample.arrays())
     dt = time.perf_counter() - t0
-    return {
+    out = {
         "value": len(sample) / dt,
         "unit": "gate-updates/s",
         "cores": threads,
@@ -253,6 +253,35 @@ def cpu_baseline(table, n, stride=8):
         "sample": f"oracle sim() of every {stride}th gate of the n={n} workload ({len(sample)} "
                   f"gates: {n_c2} SU(4), {len(sample) - n_c2} U3/CX) from |0><0| with {threads} "
                   f"OpenMP threads: {dt:.1f} s",
+    }
+    out["reference"] = reference_baseline(table, n)
+    return out
+
+
+def reference_baseline(table, n, stride=256):
+    """The reference's own CPU path (src/dmsim_cpu_omp.hpp compiled from its
+    sources into oracle/_ref, where that build exists): one run from |0><0| of
+    every stride-th gate of the workload, forward pass, its block transpose and
+    backward pass on one thread (its multi-thread path has no entry for the
+    4x4 matrices of the SU(4) gates).  None where oracle/_ref was not built."""
+    from oracle.oracle import Reference
+
+    if not Reference.available():
+        return None
+    sample = table.take(range(0, len(table), stride))
+    n_c2 = sum(1 for op in sample.ops if op == W_OP_C2)
+    t0 = time.perf_counter()
+    Reference().sim(n, sample.arrays())
+    dt = time.perf_counter() - t0
+    return {
+        "value": len(sample) / dt,
+        "unit": "gate-updates/s",
+        "cores": 1,
+        "cpu": cpu_model(),
+        "kind": "reference",
+        "sample": f"oracle/_ref (src/dmsim_cpu_omp.hpp compiled from its sources) on every "
+                  f"{stride}th gate of the n={n} workload ({len(sample)} gates: {n_c2} SU(4), "
+                  f"{len(sample) - n_c2} U3/CX) from |0><0|, one thread: {dt:.1f} s",
     }
 
 
