@@ -34,12 +34,13 @@ HBM_PEAK_GBS = 8000.0
 # measured by rocprofv3 FETCH_SIZE and WRITE_SIZE runs of this workload scaled by
 # a copy of known size; a constant here, from the summaries named beside it.
 TRAFFIC_RATIO = {
-    "k_fused4_herm": (4.364e9 / 4.343e9, "profiles/k_fused4_herm_n14_hbm_counters.md"),
-    "k_relabel": (8.594e9 / 8.590e9, "profiles/k_fused4_herm_n14_hbm_counters.md"),
+    "k_fused4_herm": (4.363e9 / 4.343e9, "profiles/k_fused4_herm_n14_hbm_counters.md"),
+    "k_relabel": (8.596e9 / 8.590e9, "profiles/k_fused4_herm_n14_hbm_counters.md"),
     "k_fused4": (8.591e9 / 8.590e9, "profiles/k_fused4_n14_hbm_counters.md"),
     "k_fused4_perm": (8.591e9 / 8.590e9, "profiles/k_fused4_n14_hbm_counters.md"),
 }
-METHODS = {"plain": 0, "plain_launch": 1, "fused": 2, "fused_learned": 2, "fused_learned5": 2}
+METHODS = {"plain": 0, "plain_launch": 1, "fused": 2, "fused_learned": 2, "fused_learned5": 2,
+           "fused_inorder": 2}  # the last only on request: in-order sweeps (profiling runs)
 W_OP_C2 = 39  # the 4x4 matrix extension op (workloads.OP["C2"])
 
 
@@ -495,6 +496,8 @@ def main():
             sim.set_cost_model(model)
         if mname == "fused_learned5":  # blocks of up to five qubits
             sim.set_fusion_max_k(5)
+        if mname == "fused_inorder":  # greedy plan without Hermitian sweeps
+            sim.set_fusion_hermitian(0)
         sim.append_table(table)
         sim.upload()
         if mname.startswith("fused"):
