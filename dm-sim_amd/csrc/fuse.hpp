@@ -27,7 +27,7 @@ struct PlanInfo {
     uint32_t perm_blocks = 0;    // fused four-qubit blocks with a trailing permutation
     uint32_t herm_blocks = 0;    // four-qubit blocks run as Hermitian sweeps
     uint32_t relabels = 0;       // relabelling sweeps
-    uint32_t grouping = 0;       // 0 least growth, 1 frontier, 2..4 frontier scored by time
+    uint32_t grouping = 0;       // 0 least growth, 1 frontier, 2..5 frontier, other scores
     uint32_t tile_hist[9] = {};  // fused blocks per tile width m (4..8)
     double predicted_us = 0.0;
     double greedy_us = 0.0;
@@ -107,10 +107,11 @@ struct BlockPlan {
 // while its target is not.
 // model: choose among the least-growth grouping, the frontier grouping and the
 // frontier grouping scored by the gates' predicted unfused times (all gates,
-// multi-qubit gates only, square roots), the one whose blocks the network
-// prices cheapest (each the cheaper of its fused sweep and its gates
-// unfused); without a model the one with fewer sweeps of the first two.
-// *grouping (may be null) = the choice: 0..4 in that order.
+// multi-qubit gates only, square roots) and by structural scores with heavy
+// one-qubit gates, the one whose blocks the network prices cheapest (each the
+// cheaper of its fused sweep and its gates unfused); without a model the one
+// with fewer sweeps of the first two.  *grouping (may be null) = the choice:
+// 0..5 in that order.
 int plan_blocks(const std::vector<Gate>& circuit, int n, int max_k, BlockPlan& out, bool& fusable,
                 bool allow_perm = false, bool herm = false, CostModel* model = nullptr,
                 uint32_t* grouping = nullptr);

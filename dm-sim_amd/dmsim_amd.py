@@ -379,7 +379,7 @@ class Sim:
                                       C.byref(gr)))
         return {"perm_blocks": pb.value, "herm_blocks": hb.value, "relabels": rl.value,
                 "grouping": ("least_growth", "frontier", "frontier_timed", "frontier_timed_multi",
-                             "frontier_timed_sqrt")[gr.value]}
+                             "frontier_timed_sqrt", "frontier_1q_heavy")[gr.value]}
 
     def time_launch(self, kind, m=0, qmask=0, target=0, control=-1, slow=0, reps=10):
         """Average device microseconds of one launch configuration (rho is left

@@ -320,8 +320,9 @@ int dmx_plan_info(const dmx_sim* sim, uint32_t* fused_blocks, uint32_t* plain_bl
  * with a trailing permutation, Hermitian block sweeps, relabelling sweeps --
  * and the grouping the planner chose (0 least growth, 1 frontier, 2 frontier
  * scored by the gates' predicted unfused times, 3 by those of the multi-qubit
- * gates only, 4 by their square roots; with a cost model the one it prices
- * cheapest, else the one with fewer sweeps of 0 and 1). */
+ * gates only, 4 by their square roots, 5 structural scores with one-qubit
+ * gates weighing four times a multi-qubit gate; with a cost model the one it
+ * prices cheapest, else the one with fewer sweeps of 0 and 1). */
 int dmx_plan_sweeps(const dmx_sim* sim, uint32_t* perm_blocks, uint32_t* herm_blocks,
                     uint32_t* relabels, uint32_t* grouping);
 
