@@ -595,7 +595,7 @@ uint64_t supertiles(int n, int m) { return (1ull << (n - 4)) * (1ull << (n - m))
 constexpr int kPipeQubits = 12;
 
 // From this many qubits the batched Hermitian sweeps pipeline their loads.
-constexpr int kBatchHermPipeQubits = 11;
+constexpr int kBatchHermPipeQubits = 10;
 
 }  // namespace
 
