@@ -334,7 +334,9 @@ def batched(steps, warmup, seed=4, n_circuits=256, trace=False):
                 "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": st["sweep_bytes"] / (st["ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS,
                 "bytes_per_run": st["sweep_bytes"], "device_ms_per_run": st["ms"],
-                "note": "rho of circuits up to 11 qubits fits the 256 MB Infinity Cache",
+                "note": "one circuit's rho (<= 67 MB at 11 qubits) fits the 256 MB Infinity "
+                        "Cache, but a launch sweeps all circuits of one width (up to 3.3 GB): "
+                        "the 10- and 11-qubit steps stream from HBM",
             }
         b.close()
     if trace:
