@@ -662,7 +662,7 @@ hipError_t Batch::launch(const Launch& l, const BatchEntry* d_entries)
 {
     const BatchEntry* e = d_entries + l.entry_off;
     switch (l.kind) {
-    case SK_HERM: return launch_fused4_herm_batch(e, l.count, l.n, kHermBatchBlocks, stream_);
+    case SK_HERM: return launch_fused4_herm_batch(e, l.count, l.n, herm_batch_blocks(l.n), stream_);
     case SK_RELABEL: return launch_relabel_batch(e, l.count, l.n, 4096, stream_);
     default: return launch_fused4_batch(e, l.count, l.n, l.m, 4096, stream_);
     }

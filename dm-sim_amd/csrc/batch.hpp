@@ -100,9 +100,11 @@ public:
     // Workgroups of one batched Hermitian launch in all (shared among its
     // circuits), so that each takes several upper-tile items in turn and its
     // next item's loads overlap the transform.  256 hardware-efficient
-    // circuits of 6..11 qubits: 61.2 ms per run at 2048, 62.0 at 1536, 63.0 at
-    // 3072, 64.8 at 4096 (the in-order and relabel launches are indifferent).
-    static constexpr int kHermBatchBlocks = 2048;
+    // circuits of 6..11 qubits, one count for both widths: 61.2 ms per run at
+    // 2048, 62.0 at 1536, 63.0 at 3072, 64.8 at 4096 (the in-order and relabel
+    // launches are indifferent); then 10-qubit launches alone: 59.5 ms at
+    // 2048, 58.9 at 1536 and at 1024, 60.9 at 3072.
+    static constexpr int herm_batch_blocks(int n) { return n >= 11 ? 2048 : 1536; }
 
     double last_ms() const { return last_ms_; }
     uint32_t launches() const

@@ -254,6 +254,32 @@ def test_learned_plan_five_qubit_gate_at_max_k4(dm, costdata, herm):
     assert max(np.abs(re - re0).max(), np.abs(im - im0).max()) <= 1e-10
 
 
+def test_learned_plan_is_deterministic(dm, costdata):
+    """The learned planner builds its candidate groupings on host threads of
+    their own: two uploads of one circuit give the same plan (grouping, sweep
+    counts, predicted time) and bit-identical rho."""
+    import workloads as W
+    model = costdata.default_model()
+    n = 12
+    table = W.brickwall_su4(n=n, layers=6, tail=300, seed=5)
+    print(f"cost: learned plan twice, n={n}, {len(table)} gates", flush=True)
+    plans, states = [], []
+    for _ in range(2):
+        s = dm.Sim(n)
+        s.set_method(dm.METHOD_FUSED)
+        s.set_cost_model(model)
+        s.append_table(table)
+        s.upload()
+        plans.append((s.plan_sweeps(), s.plan_info()))
+        s.run()
+        states.append(s.get_dm())
+        s.close()
+    assert plans[0] == plans[1]
+    assert plans[0][0]["herm_blocks"] > 0
+    assert np.array_equal(states[0][0], states[1][0])
+    assert np.array_equal(states[0][1], states[1][1])
+
+
 def _qasm(name):
     here = os.path.dirname(os.path.abspath(__file__))
     with open(os.path.join(here, "golden", "qasm", name)) as f:
