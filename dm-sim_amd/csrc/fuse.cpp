@@ -1238,6 +1238,7 @@ int choose_grouping(const std::vector<Gate>& circuit, int n, int kmax, bool allo
     const std::vector<long>* wts[kCands] = {nullptr, nullptr, &w, &wm, &ws, &wh};
     std::exception_ptr err[kCands];
     std::vector<std::thread> pool;
+    pool.reserve(kCands - 1);  // no reallocation (and no bad_alloc) once threads run
     for (int c = 1; c < kCands; c++) {
         auto job = [&, c] {
             try {
