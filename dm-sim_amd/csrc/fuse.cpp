@@ -1450,8 +1450,9 @@ int choose_fused5(const std::vector<Gate>& circuit, int n, BlockPlan& bp, CostMo
 // spanning pi(span Q) at bits 0..3 (ascending), then the images of the
 // super-tile's four low bits, then the rest by their current bit.  Its
 // record gets the block's bits and matrix in the current order and the map
-// phi = sigma' o pi o sigma^-1 on index bits.  Before any other step and at
-// the end, a relabelling step restores the circuit's order.
+// phi = sigma' o pi o sigma^-1 on index bits.  A five-qubit block runs in
+// order on the bits that hold its qubits, in any order.  Before any other
+// step and at the end, a relabelling step restores the circuit's order.
 void to_hermitian(Plan& plan, int n, PlanInfo& info)
 {
     std::vector<Step> steps;
@@ -1474,6 +1475,35 @@ void to_hermitian(Plan& plan, int n, PlanInfo& info)
         bo.go_home();
     };
     for (const Step& st : plan.steps) {
+        if (st.type == ST_FUSED5 && !bo.home()) {
+            // an in-order five-qubit sweep runs on whatever bits hold its
+            // qubits: its record gets them and its matrix in the current
+            // order, and the order stays (no relabelling before or after)
+            FusedBlock5 h = plan.blocks5[st.index];
+            const uint32_t qphys = bo.phys(h.qmask);
+            int tphys[5], t = 0;
+            for (uint32_t m = h.qmask; m; m &= m - 1)
+                tphys[t++] = __builtin_popcount(qphys & ((1u << bo.sigma[__builtin_ctz(m)]) - 1u));
+            auto remap = [&](int a) {
+                int o = 0;
+                for (int u = 0; u < 5; u++)
+                    if ((a >> u) & 1) o |= 1 << tphys[u];
+                return o;
+            };
+            const FusedBlock5& b = plan.blocks5[st.index];
+            for (int a = 0; a < 32; a++)
+                for (int c = 0; c < 32; c++) {
+                    h.ur[remap(a) * 32 + remap(c)] = b.ur[a * 32 + c];
+                    h.ui[remap(a) * 32 + remap(c)] = b.ui[a * 32 + c];
+                }
+            h.qmask = qphys;
+            if (!tile_bits5_ok(n, (int)h.m, qphys))
+                h.m = (uint32_t)(tile_bits5_ok(n, default_tile_bits5(n), qphys) ? default_tile_bits5(n)
+                                                                                 : std::min(8, n));
+            steps.push_back({ST_FUSED5, (int)plan.blocks5.size(), 0});
+            plan.blocks5.push_back(h);
+            continue;
+        }
         if (st.type != ST_FUSED4) {
             go_home();
             steps.push_back(st);

@@ -29,8 +29,10 @@ static int fail(const char* what, int n, int seed)
 }
 
 // Brick-wall layers of random two-qubit matrices, then one- and two-qubit
-// gates with many X, CX and SWAP (trailing index maps), then a few wider ones.
-static std::vector<Gate> circuit(int n, int seed)
+// gates with many X, CX and SWAP (trailing index maps), then a few wider ones;
+// with c4x, a five-qubit gate halfway through those (an in-order five-qubit
+// step between Hermitian sweeps).
+static std::vector<Gate> circuit(int n, int seed, bool c4x = false)
 {
     static const int kOps[] = {dmx::OP_U3, dmx::OP_RZ, dmx::OP_H, dmx::OP_X, dmx::OP_X,
                                dmx::OP_CX, dmx::OP_CX, dmx::OP_CX, dmx::OP_SWAP, dmx::OP_SWAP,
@@ -66,7 +68,17 @@ static std::vector<Gate> circuit(int n, int seed)
             for (int i = 0; i < 16; i++) m.mre[i] = M[i].real(), m.mim[i] = M[i].imag();
             c.push_back(m);
         }
-    for (int i = 0; i < 90 + 10 * n; i++) {
+    const int n_rand = 90 + 10 * n;
+    for (int i = 0; i < n_rand; i++) {
+        if (c4x && i == n_rand / 2) {
+            std::vector<int> q(n);
+            for (int t = 0; t < n; t++) q[t] = t;
+            std::shuffle(q.begin(), q.end(), rng);
+            Gate x;
+            x.op = dmx::OP_C4X;
+            for (int t = 0; t < 5; t++) x.qb[t] = (uint32_t)q[t];
+            c.push_back(x);
+        }
         Gate x;
         x.op = kOps[rng() % kNum];
         x.theta = ang(rng), x.phi = ang(rng), x.lambda = ang(rng);
@@ -114,6 +126,38 @@ static void apply_gate(const Gate& g, int n, std::vector<cd>& S)
     };
     for (uint64_t r = 0; r < dim; r++) vec(1, r * dim, false);   // each row: M
     for (uint64_t c = 0; c < dim; c++) vec(dim, c, true);        // each column: conj(M)
+}
+
+// One in-order five-qubit step (k_fused5) in place: S <- conj(U) S U^T on
+// the index bits of blk.qmask, local bit t of U on its t-th lowest bit.
+static void run_fused5(const dmx::FusedBlock5& blk, int n, std::vector<cd>& S)
+{
+    const uint64_t dim = 1ull << n;
+    int qs[5], t = 0;
+    for (uint32_t m = blk.qmask; m; m &= m - 1) qs[t++] = __builtin_ctz(m);
+    auto dep = [&](int l) {
+        uint64_t x = 0;
+        for (int u = 0; u < 5; u++)
+            if ((l >> u) & 1) x |= 1ull << qs[u];
+        return x;
+    };
+    cd in[32];
+    auto vec = [&](uint64_t stride, uint64_t offset, bool conj) {
+        for (uint64_t base = 0; base < dim; base++) {
+            if (base & blk.qmask) continue;
+            for (int l = 0; l < 32; l++) in[l] = S[offset + (base | dep(l)) * stride];
+            for (int r = 0; r < 32; r++) {
+                cd s = 0.0;
+                for (int l = 0; l < 32; l++) {
+                    const cd u(blk.ur[r * 32 + l], blk.ui[r * 32 + l]);
+                    s += (conj ? std::conj(u) : u) * in[l];
+                }
+                S[offset + (base | dep(r)) * stride] = s;
+            }
+        }
+    };
+    for (uint64_t r = 0; r < dim; r++) vec(1, r * dim, false);  // each row: U
+    for (uint64_t c = 0; c < dim; c++) vec(dim, c, true);       // each column: conj(U)
 }
 
 // One k_fused4_herm launch, thread by thread.
@@ -243,8 +287,10 @@ int main()
 {
     int cases = 0;
     for (int n = 8; n <= 10; n++)
-        for (int seed = 0; seed < (n == 8 ? 3 : n == 9 ? 2 : 1); seed++) {
-            const std::vector<Gate> c = circuit(n, 100 * n + seed);
+        for (int seed = 0; seed < (n == 8 ? 5 : n == 9 ? 3 : 2); seed++) {
+            // the last seed of each width: with a C4X halfway
+            const bool c4x = seed == (n == 8 ? 4 : n == 9 ? 2 : 1);
+            const std::vector<Gate> c = circuit(n, 100 * n + seed, c4x);
             dmx::FusedPlan* plan = nullptr;
             dmx::PlanInfo info;
             if (dmx::build_fused_plan(c, n, 4, nullptr, 0, true, &plan, &info, true) != 0 || !plan)
@@ -253,10 +299,17 @@ int main()
             plan->describe(steps);
             if (info.herm_blocks == 0 || info.relabels != 1 || info.perm_blocks == 0)
                 return fail("plan has no Hermitian sweeps, relabel or trailing maps", n, seed);
-            int halves = 0;
-            for (const auto& st : steps) halves += st.kind == dmx::PlanStepView::HERM && st.block->next_q;
-            if (halves + 1 != (int)info.herm_blocks)
-                return fail("every Hermitian sweep but the last writes half", n, seed);
+            // a Hermitian sweep writes half exactly when another follows it
+            int fused5 = 0;
+            for (size_t i = 0; i < steps.size(); i++) {
+                fused5 += steps[i].kind == dmx::PlanStepView::FUSED5;
+                if (steps[i].kind != dmx::PlanStepView::HERM) continue;
+                const bool next_herm = i + 1 < steps.size() && steps[i + 1].kind == dmx::PlanStepView::HERM;
+                if ((steps[i].block->next_q != 0) != next_herm)
+                    return fail("a Hermitian sweep writes half unless a non-Hermitian step follows", n, seed);
+            }
+            if (c4x != (fused5 == 1))
+                return fail("the C4X is not the plan's one five-qubit step", n, seed);
             const uint64_t dim = 1ull << n;
             std::mt19937 rng(17u + (uint32_t)seed);
             std::normal_distribution<double> g(0.0, 1.0);
@@ -275,6 +328,9 @@ int main()
                     if (!run_herm(*st.block, n, cur, nxt, &why)) return fail(why, n, seed);
                 } else if (st.kind == dmx::PlanStepView::RELABEL) {
                     if (!run_relabel(*st.block, n, cur, nxt, &why)) return fail(why, n, seed);
+                } else if (st.kind == dmx::PlanStepView::FUSED5) {
+                    nxt = cur;
+                    run_fused5(*st.block5, n, nxt);
                 } else {
                     return fail("unexpected step kind in a Hermitian plan", n, seed);
                 }

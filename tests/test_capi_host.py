@@ -144,9 +144,11 @@ def test_planner_grouping_is_valid(tmp_path):
 def test_hermitian_sweep_maps(tmp_path):
     """The Hermitian fused executor's index maps, executed on the CPU thread by
     thread with the kernels' own map functions (dm-sim_amd/csrc/herm.hpp) on
-    plans from dmx::build_fused_plan at n = 8..10: every load and store inside
-    the planes, every entry written once per sweep, and the result equal to
-    the circuit applied gate by gate (tests/cpp/herm_maps.cpp)."""
+    plans from dmx::build_fused_plan at n = 8..10, one circuit per width with a
+    C4X (an in-order five-qubit step on whatever index bits hold its qubits):
+    every load and store inside the planes, every entry written once per
+    sweep, one relabelling per plan, and the result equal to the circuit
+    applied gate by gate (tests/cpp/herm_maps.cpp)."""
     import shutil
     import subprocess
     hipcc = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
@@ -160,4 +162,4 @@ def test_hermitian_sweep_maps(tmp_path):
     r = subprocess.run(cmd, capture_output=True, text=True)
     assert r.returncode == 0, r.stderr
     r = subprocess.run([exe], capture_output=True, text=True, timeout=300)
-    assert r.returncode == 0 and r.stdout.strip() == "6 cases OK", r.stdout
+    assert r.returncode == 0 and r.stdout.strip() == "10 cases OK", r.stdout
