@@ -1559,6 +1559,41 @@ void to_hermitian(Plan& plan, int n, PlanInfo& info)
     plan.steps.swap(steps);
 }
 
+// The width of every five-qubit sweep of a plan priced by the network on the
+// index bits it runs on (a Hermitian plan moves them; the grouping priced
+// the circuit's own).
+int reprice_fused5_widths(Plan& plan, int n, CostModel& model)
+{
+    const int m_hi = std::min(8, n);
+    std::vector<std::vector<float>> feats;
+    std::vector<int> which;
+    float f[kMlpFeatures];
+    for (const Step& st : plan.steps) {
+        if (st.type != ST_FUSED5) continue;
+        which.push_back(st.index);
+        for (int m = 5; m <= m_hi; m++) {
+            // an inadmissible width is priced like a valid one and never chosen
+            const int mm = tile_bits5_ok(n, m, plan.blocks5[st.index].qmask) ? m : m_hi;
+            cost_features(n, CK_FUSED5, mm, plan.blocks5[st.index].qmask, 0, -1, 0, f);
+            feats.emplace_back(f, f + kMlpFeatures);
+        }
+    }
+    if (which.empty()) return DMX_OK;
+    std::vector<float> us;
+    const int rc = model.price(feats, us);
+    if (rc) return rc;
+    const int n_m = m_hi - 5 + 1;
+    for (size_t i = 0; i < which.size(); i++) {
+        FusedBlock5& b = plan.blocks5[which[i]];
+        const float* t = us.data() + i * n_m;
+        int best_m = (int)b.m;
+        for (int k = 0; k < n_m; k++)
+            if (tile_bits5_ok(n, 5 + k, b.qmask) && t[k] < t[best_m - 5]) best_m = 5 + k;
+        b.m = (uint32_t)best_m;
+    }
+    return DMX_OK;
+}
+
 }  // namespace
 
 int build_fused_plan(const std::vector<Gate>& circuit, int n, int max_k, CostModel* model,
@@ -1627,6 +1662,13 @@ int build_fused_plan(const std::vector<Gate>& circuit, int n, int max_k, CostMod
         }
     }
     if (herm) to_hermitian(*plan, n, pi);
+    if (herm && model && tile_bits <= 0) {
+        rc = reprice_fused5_widths(*plan, n, *model);
+        if (rc) {
+            delete plan;
+            return rc;
+        }
+    }
     if (model && tile_bits <= 0 && max_k > 4 && !plan->blocks5.empty()) {
         // Whole-plan check: five-qubit blocks pay off only when they replace
         // enough four-qubit sweeps, which no single block's split can show.
