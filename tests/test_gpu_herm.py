@@ -118,9 +118,10 @@ def test_herm_after_plain_run(oracle):
 
 
 def test_herm_with_channels_and_five_qubit_blocks():
-    """Channels and five-qubit blocks (C4X) run in the circuit's bit order:
-    the plan relabels before each of them.  Against the plain method, over
-    two runs."""
+    """Channels run in the circuit's bit order (the plan relabels before each
+    of them); a five-qubit block (C4X), here after Hermitian sweeps, runs on
+    whatever index bits hold its qubits.  Against the plain method, over two
+    runs."""
     dm = dmx()
     n = 10
     rng = np.random.default_rng(9400)
