@@ -1450,9 +1450,10 @@ int choose_fused5(const std::vector<Gate>& circuit, int n, BlockPlan& bp, CostMo
 // spanning pi(span Q) at bits 0..3 (ascending), then the images of the
 // super-tile's four low bits, then the rest by their current bit.  Its
 // record gets the block's bits and matrix in the current order and the map
-// phi = sigma' o pi o sigma^-1 on index bits.  A five-qubit block runs in
-// order on the bits that hold its qubits, in any order.  Before any other
-// step and at the end, a relabelling step restores the circuit's order.
+// phi = sigma' o pi o sigma^-1 on index bits.  A five-qubit block and a
+// channel run in order on the bits that hold their qubits, in any order.
+// Before any other step and at the end, a relabelling step restores the
+// circuit's order.
 void to_hermitian(Plan& plan, int n, PlanInfo& info)
 {
     std::vector<Step> steps;
@@ -1502,6 +1503,20 @@ void to_hermitian(Plan& plan, int n, PlanInfo& info)
                                                                                  : std::min(8, n));
             steps.push_back({ST_FUSED5, (int)plan.blocks5.size(), 0});
             plan.blocks5.push_back(h);
+            continue;
+        }
+        if (st.type == ST_CHANNEL && !bo.home()) {
+            // a channel's two records (one per orientation) act on the same
+            // row and column bit, which a relabelling moves alike: they get
+            // the bit that holds their qubit, and the order stays
+            const int first = (int)plan.prims.size();
+            for (int t = 0; t < 2; t++) {
+                PrimOp p = plan.prims[st.index + t];
+                p.q0 = bo.sigma[p.q0];
+                p.q1 = bo.sigma[p.q1];
+                plan.prims.push_back(p);
+            }
+            steps.push_back({ST_CHANNEL, first, 2});
             continue;
         }
         if (st.type != ST_FUSED4) {
