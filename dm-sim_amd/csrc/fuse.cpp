@@ -166,11 +166,13 @@ struct Plan : FusedPlan {
                      : st.type == ST_RELABEL ? PlanStepView::RELABEL
                      : st.type == ST_FUSED4  ? PlanStepView::FUSED4
                      : st.type == ST_FUSED5  ? PlanStepView::FUSED5
+                     : st.type == ST_CHANNEL ? PlanStepView::CHANNEL
                                              : PlanStepView::OTHER;
             v.block = (st.type == ST_HERM || st.type == ST_RELABEL || st.type == ST_FUSED4)
                           ? &blocks[st.index]
                           : nullptr;
             v.block5 = st.type == ST_FUSED5 ? &blocks5[st.index] : nullptr;
+            v.prim = st.type == ST_CHANNEL ? &prims[st.index] : nullptr;
             out.push_back(v);
         }
     }

@@ -35,9 +35,11 @@ struct PlanInfo {
 
 // One step of a fused plan as the executor runs it (host-side checks).
 struct PlanStepView {
-    enum Kind { FUSED4, FUSED5, HERM, RELABEL, OTHER } kind;
+    enum Kind { FUSED4, FUSED5, HERM, RELABEL, CHANNEL, OTHER } kind;
     const FusedBlock* block;    // FUSED4, HERM, RELABEL: its record as launched
     const FusedBlock5* block5;  // FUSED5
+    const PrimOp* prim;         // CHANNEL: its record for the stored orientation (the
+                                // conj-transposed one follows it)
 };
 
 // Algorithmic bytes of one Hermitian block sweep (k_fused4_herm) on n >= 8

@@ -2,9 +2,11 @@
 // with Hermitian sweeps (dmx::build_fused_plan, herm = true) are executed on
 // the CPU step by step with the same per-thread load/store maps the kernels
 // use (herm.hpp: herm_load/herm_direct/herm_mirror for k_fused4_herm,
-// relabel_load/relabel_store for k_relabel), on a random Hermitian stored
+// relabel_load/relabel_store for k_relabel; five-qubit blocks and channels
+// in place on the index bits their records name), on a random Hermitian stored
 // matrix S, and compared with S <- conj(M) S M^T applied gate by gate with the
-// gates' own matrices (dmx::gate_matrix).  Every store must stay inside the
+// gates' own matrices (dmx::gate_matrix; channels as their Kraus sums
+// straight from the operators).  Every store must stay inside the
 // planes, every output entry must be written exactly once per step, and the
 // final S must agree to 1e-12.  Prints "<cases> cases OK", or the first
 // violation and exits nonzero.
@@ -28,11 +30,49 @@ static int fail(const char* what, int n, int seed)
     return 1;
 }
 
+// A channel's Kraus operators: the two 2x2 halves of a random 4x2 isometry V
+// (sum K^dagger K = V^dagger V = I).
+struct Channel {
+    uint32_t q;
+    cd K[2][4];
+};
+
+// The channel as the engine stores it: OP_KRAUS with the superoperator on the
+// entry quad (row bit a, column bit b) of the stored A = conj(rho),
+// S[(a,b),(a',b')] = sum_i conj(K_i[a][a']) K_i[b][b'], then its twin for the
+// conj-transposed orientation.
+static Gate kraus_gate(const Channel& ch)
+{
+    Gate g;
+    g.op = dmx::OP_KRAUS;
+    g.k = 1;
+    g.qb[0] = ch.q;
+    g.mre.assign(32, 0.0);
+    g.mim.assign(32, 0.0);
+    cd S[16];
+    for (int idx = 0; idx < 16; idx++) {
+        const int a = idx >> 3, b = (idx >> 2) & 1, a2 = (idx >> 1) & 1, b2 = idx & 1;
+        S[idx] = 0.0;
+        for (int i = 0; i < 2; i++) S[idx] += std::conj(ch.K[i][2 * a + a2]) * ch.K[i][2 * b + b2];
+    }
+    for (int idx = 0; idx < 16; idx++) {
+        const int a = idx >> 3, b = (idx >> 2) & 1, a2 = (idx >> 1) & 1, b2 = idx & 1;
+        const int tdx = (2 * b + a) * 4 + (2 * b2 + a2);
+        g.mre[idx] = S[idx].real();
+        g.mim[idx] = S[idx].imag();
+        g.mre[16 + idx] = S[tdx].real();
+        g.mim[16 + idx] = -S[tdx].imag();
+    }
+    return g;
+}
+
 // Brick-wall layers of random two-qubit matrices, then one- and two-qubit
 // gates with many X, CX and SWAP (trailing index maps), then a few wider ones;
 // with c4x, a five-qubit gate halfway through those (an in-order five-qubit
-// step between Hermitian sweeps).
-static std::vector<Gate> circuit(int n, int seed, bool c4x = false)
+// step between Hermitian sweeps) and a random channel a quarter and three
+// quarters of the way (their Kraus operators in chans).
+static std::vector<Gate> circuit(int n, int seed, bool c4x = false,
+                                 std::vector<Channel>* chans = nullptr)
 {
     static const int kOps[] = {dmx::OP_U3, dmx::OP_RZ, dmx::OP_H, dmx::OP_X, dmx::OP_X,
                                dmx::OP_CX, dmx::OP_CX, dmx::OP_CX, dmx::OP_SWAP, dmx::OP_SWAP,
@@ -70,6 +110,27 @@ static std::vector<Gate> circuit(int n, int seed, bool c4x = false)
         }
     const int n_rand = 90 + 10 * n;
     for (int i = 0; i < n_rand; i++) {
+        if (c4x && chans && (i == n_rand / 4 || i == 3 * n_rand / 4)) {
+            Channel ch;
+            ch.q = (uint32_t)(rng() % n);
+            cd V[4][2];
+            for (int col = 0; col < 2; col++) {
+                for (int r = 0; r < 4; r++) V[r][col] = cd(g(rng), g(rng));
+                for (int p = 0; p < col; p++) {
+                    cd dot = 0.0;
+                    for (int r = 0; r < 4; r++) dot += std::conj(V[r][p]) * V[r][col];
+                    for (int r = 0; r < 4; r++) V[r][col] -= dot * V[r][p];
+                }
+                double nrm = 0;
+                for (int r = 0; r < 4; r++) nrm += std::norm(V[r][col]);
+                for (int r = 0; r < 4; r++) V[r][col] /= std::sqrt(nrm);
+            }
+            for (int i2 = 0; i2 < 2; i2++)
+                for (int r = 0; r < 2; r++)
+                    for (int col = 0; col < 2; col++) ch.K[i2][2 * r + col] = V[2 * i2 + r][col];
+            chans->push_back(ch);
+            c.push_back(kraus_gate(ch));
+        }
         if (c4x && i == n_rand / 2) {
             std::vector<int> q(n);
             for (int t = 0; t < n; t++) q[t] = t;
@@ -126,6 +187,49 @@ static void apply_gate(const Gate& g, int n, std::vector<cd>& S)
     };
     for (uint64_t r = 0; r < dim; r++) vec(1, r * dim, false);   // each row: M
     for (uint64_t c = 0; c < dim; c++) vec(dim, c, true);        // each column: conj(M)
+}
+
+// S <- sum_i conj(K_i) S K_i^T on qubit ch.q (rows with conj(K_i), columns
+// with K_i), straight from the Kraus operators.
+static void apply_channel(const Channel& ch, int n, std::vector<cd>& S)
+{
+    const uint64_t dim = 1ull << n, bit = 1ull << ch.q;
+    for (uint64_t x0 = 0; x0 < dim; x0++) {
+        if (x0 & bit) continue;
+        for (uint64_t y0 = 0; y0 < dim; y0++) {
+            if (y0 & bit) continue;
+            cd in[2][2], out[2][2] = {};
+            for (int a = 0; a < 2; a++)
+                for (int b = 0; b < 2; b++) in[a][b] = S[(x0 | (a ? bit : 0)) * dim + (y0 | (b ? bit : 0))];
+            for (int i = 0; i < 2; i++)
+                for (int a = 0; a < 2; a++)
+                    for (int b = 0; b < 2; b++)
+                        for (int a2 = 0; a2 < 2; a2++)
+                            for (int b2 = 0; b2 < 2; b2++)
+                                out[a][b] += std::conj(ch.K[i][2 * a + a2]) * in[a2][b2] *
+                                             ch.K[i][2 * b + b2];
+            for (int a = 0; a < 2; a++)
+                for (int b = 0; b < 2; b++) S[(x0 | (a ? bit : 0)) * dim + (y0 | (b ? bit : 0))] = out[a][b];
+        }
+    }
+}
+
+// One channel step (k_super) in place: the record's 4x4 on the entry quad of
+// row bit q0 and column bit q0, quad index 2 * (row bit) + (column bit).
+static void run_channel(const dmx::PrimOp& p, int n, std::vector<cd>& S)
+{
+    const uint64_t dim = 1ull << n, bit = 1ull << p.q0;
+    for (uint64_t x0 = 0; x0 < dim; x0++) {
+        if (x0 & bit) continue;
+        for (uint64_t y0 = 0; y0 < dim; y0++) {
+            if (y0 & bit) continue;
+            cd in[4], out[4] = {};
+            for (int j = 0; j < 4; j++) in[j] = S[(x0 | ((j >> 1) ? bit : 0)) * dim + (y0 | ((j & 1) ? bit : 0))];
+            for (int i = 0; i < 4; i++)
+                for (int j = 0; j < 4; j++) out[i] += cd(p.c[2 * (4 * i + j)], p.c[2 * (4 * i + j) + 1]) * in[j];
+            for (int i = 0; i < 4; i++) S[(x0 | ((i >> 1) ? bit : 0)) * dim + (y0 | ((i & 1) ? bit : 0))] = out[i];
+        }
+    }
 }
 
 // One in-order five-qubit step (k_fused5) in place: S <- conj(U) S U^T on
@@ -290,7 +394,8 @@ int main()
         for (int seed = 0; seed < (n == 8 ? 5 : n == 9 ? 3 : 2); seed++) {
             // the last seed of each width: with a C4X halfway
             const bool c4x = seed == (n == 8 ? 4 : n == 9 ? 2 : 1);
-            const std::vector<Gate> c = circuit(n, 100 * n + seed, c4x);
+            std::vector<Channel> chans;
+            const std::vector<Gate> c = circuit(n, 100 * n + seed, c4x, &chans);
             dmx::FusedPlan* plan = nullptr;
             dmx::PlanInfo info;
             if (dmx::build_fused_plan(c, n, 4, nullptr, 0, true, &plan, &info, true) != 0 || !plan)
@@ -310,6 +415,9 @@ int main()
             }
             if (c4x != (fused5 == 1))
                 return fail("the C4X is not the plan's one five-qubit step", n, seed);
+            int n_chan = 0;
+            for (const auto& st : steps) n_chan += st.kind == dmx::PlanStepView::CHANNEL;
+            if (n_chan != (int)chans.size()) return fail("a channel is not a step of its own", n, seed);
             const uint64_t dim = 1ull << n;
             std::mt19937 rng(17u + (uint32_t)seed);
             std::normal_distribution<double> g(0.0, 1.0);
@@ -321,7 +429,13 @@ int main()
                     S[j * dim + i] = std::conj(v);
                 }
             std::vector<cd> want = S, cur = S, nxt(dim * dim);
-            for (const Gate& gt : c) apply_gate(gt, n, want);
+            size_t next_chan = 0;
+            for (const Gate& gt : c) {
+                if (gt.op == dmx::OP_KRAUS)
+                    apply_channel(chans[next_chan++], n, want);
+                else
+                    apply_gate(gt, n, want);
+            }
             for (const auto& st : steps) {
                 const char* why = "";
                 if (st.kind == dmx::PlanStepView::HERM) {
@@ -331,6 +445,9 @@ int main()
                 } else if (st.kind == dmx::PlanStepView::FUSED5) {
                     nxt = cur;
                     run_fused5(*st.block5, n, nxt);
+                } else if (st.kind == dmx::PlanStepView::CHANNEL) {
+                    nxt = cur;
+                    run_channel(*st.prim, n, nxt);
                 } else {
                     return fail("unexpected step kind in a Hermitian plan", n, seed);
                 }

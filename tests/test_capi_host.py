@@ -145,10 +145,11 @@ def test_hermitian_sweep_maps(tmp_path):
     """The Hermitian fused executor's index maps, executed on the CPU thread by
     thread with the kernels' own map functions (dm-sim_amd/csrc/herm.hpp) on
     plans from dmx::build_fused_plan at n = 8..10, one circuit per width with a
-    C4X (an in-order five-qubit step on whatever index bits hold its qubits):
-    every load and store inside the planes, every entry written once per
-    sweep, one relabelling per plan, and the result equal to the circuit
-    applied gate by gate (tests/cpp/herm_maps.cpp)."""
+    C4X and two channels with random Kraus operators (in-order steps on
+    whatever index bits hold their qubits): every load and store inside the
+    planes, every entry written once per sweep, one relabelling per plan, and
+    the result equal to the circuit applied gate by gate, channels as their
+    Kraus sums (tests/cpp/herm_maps.cpp)."""
     import shutil
     import subprocess
     hipcc = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
