@@ -1155,18 +1155,16 @@ int price_grouping(const std::vector<Gate>& circuit, const std::vector<PB>& pbs,
     float f[kMlpFeatures];
     BitOrder bo(n);
     for (const auto& e : seq) {
-        if (e.first != 0) {  // a channel: rho returns to the circuit's order first
-            bo.go_home();
-            continue;
-        }
+        if (e.first != 0) continue;  // a channel: on the bit that holds its qubit, order kept
         const PB& pb = pbs[e.second];
         uint32_t q = pb.mask;
         for (int b = 0; b < n && __builtin_popcount(q) < 4; b++) q |= 1u << b;
         if (__builtin_popcount(q) == 5) {
+            // in order on the bits that hold its qubits, order kept
+            const uint32_t qp = bo.phys(q);
             int m = default_tile_bits5(n);
-            if (!tile_bits5_ok(n, m, q)) m = std::min(8, n);
-            cost_features(n, CK_FUSED5, m, q, 0, -1, 0, f);
-            bo.go_home();
+            if (!tile_bits5_ok(n, m, qp)) m = std::min(8, n);
+            cost_features(n, CK_FUSED5, m, qp, 0, -1, 0, f);
         } else if (herm) {
             // priced on the index bits the sweep reads the block on, after
             // the relabelling of the Hermitian sweeps before it
@@ -1298,10 +1296,7 @@ int choose_executions(const std::vector<Gate>& circuit, int n, BlockPlan& bp, Co
     if (herm) {
         BitOrder bo(n);
         for (const auto& o : bp.order) {
-            if (o.first != 0) {
-                bo.go_home();
-                continue;
-            }
+            if (o.first != 0) continue;  // five-qubit blocks and channels keep the order
             const FusedBlock& fb = bp.blocks[o.second];
             uint32_t Lp[16];
             for (int q = 0; q < 16; q++) Lp[q] = q < n ? (fb.perm_on ? fb.perm_L[q] : 1u << q) : 0u;
